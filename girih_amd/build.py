@@ -1,0 +1,81 @@
+"""Build the HIP library (gfx950) and the C driver in-tree.
+
+    python -m girih_amd.build            # libgirih_gpu.so + mwd_kernel_gpu
+    python -m girih_amd.build --lib      # library only
+
+Outputs: girih_amd/lib/libgirih_gpu.so, girih_amd/driver/bin/mwd_kernel_gpu.
+"""
+import argparse
+import os
+import shutil
+import subprocess
+import sys
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(PKG)
+CSRC = os.path.join(PKG, "csrc")
+LIBDIR = os.path.join(PKG, "lib")
+LIB = os.path.join(LIBDIR, "libgirih_gpu.so")
+DRIVER_DIR = os.path.join(PKG, "driver")
+DRIVER_BIN = os.path.join(DRIVER_DIR, "bin", "mwd_kernel_gpu")
+
+HIP_SOURCES = ["gr_runtime.hip", "gr_tblock.hip"]
+ARCH = os.environ.get("GIRIH_OFFLOAD_ARCH", "gfx950")
+
+# -ffp-contract=off: the reference-order kernels must not fuse multiply-adds
+# (bit-exact FP64/FP32 against girih's serial reference); FAST kernels call fma
+# explicitly.  packed-fp32-ops is disabled for all device code.
+HIPCC_FLAGS = ["-O3", "-std=c++17", "--offload-arch=" + ARCH, "-fPIC", "-ffp-contract=off",
+               "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+               "-Wno-unused-command-line-argument", "-munsafe-fp-atomics"]
+
+
+def hipcc():
+    for c in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", shutil.which("hipcc")):
+        if c and os.path.exists(c):
+            return c
+    raise RuntimeError("hipcc not found")
+
+
+def run(cmd):
+    print("+", " ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+
+
+def build_lib(verbose=False):
+    os.makedirs(LIBDIR, exist_ok=True)
+    objs = []
+    for src in HIP_SOURCES:
+        path = os.path.join(CSRC, src)
+        if not os.path.exists(path):
+            continue
+        obj = os.path.join(LIBDIR, os.path.splitext(src)[0] + ".o")
+        run([hipcc()] + HIPCC_FLAGS + ["-I" + os.path.join(REPO, "include"), "-c", path, "-o", obj])
+        objs.append(obj)
+    tmp = LIB + ".tmp"
+    run([hipcc(), "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs)
+    os.replace(tmp, LIB)
+    return LIB
+
+
+def build_driver():
+    src = os.path.join(DRIVER_DIR, "mwd_kernel_gpu.c")
+    if not os.path.exists(src):
+        return None
+    os.makedirs(os.path.dirname(DRIVER_BIN), exist_ok=True)
+    run(["gcc", "-O2", "-std=c99", "-Wall", "-I" + os.path.join(REPO, "include"), "-o", DRIVER_BIN, src,
+         "-L" + LIBDIR, "-lgirih_gpu", "-Wl,-rpath,$ORIGIN/../../lib", "-lm"])
+    return DRIVER_BIN
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--lib", action="store_true", help="library only")
+    args = ap.parse_args(argv)
+    build_lib()
+    if not args.lib:
+        build_driver()
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,33 @@
+// gr_internal.hpp -- types shared by the host-side translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+
+namespace gr {
+
+struct Geom;
+
+// One problem's geometry and device fields (see gr_runtime.hip for the layout).
+struct Problem {
+  int kernel = 0, dtype = 0, flags = 0;
+  int nx = 0, ny = 0, nz = 0, R = 0, ncoef = 0;
+  int xoff = 0, pitch = 0, rows = 0, planes = 0;
+  long long field_elems = 0;  // elements of one field allocation
+  double c[5] = {0, 0, 0, 0, 0};
+  double roc2s = 0;
+  void *roc2 = nullptr;  // device roc2 field base (x offset applied)
+  void *coef = nullptr;  // device coefficient arrays base (x offset applied)
+};
+
+struct LaunchCfg {
+  int zchunk = 0;   // z planes per workgroup (0 = whole box)
+  int t_block = 1;  // fused time steps per pass
+};
+
+size_t elem_size(int dtype);
+int set_error(int code, const char *msg);
+hipError_t sweep_fields(const Problem &P, void *u, const void *v, const void *src, long long src_idx,
+                        const LaunchCfg &lc, hipStream_t st);
+hipError_t sweep_box(const Problem &P, const Geom &g, void *u, const void *v, const LaunchCfg &lc, hipStream_t st);
+
+}  // namespace gr

@@ -1,0 +1,685 @@
+// gr_runtime.hip -- host side of the C ABI declared in include/girih_gpu.h.
+//
+// Owns the device fields of one problem, converts between girih's padded host
+// layout (src/utils.c:367-374) and the device layout, and drives the sweep
+// kernels with girih's step pairing (src/kernels/nb_naive_ts.c:187-203).
+//
+// Device layout of one field: element (i, j, k) (halo-inclusive girih indices)
+// at  base + x_offset + (k*rows_y + j)*pitch_x + i, rows_y = ny+2R,
+// planes = nz+2R.  x_offset puts the first interior cell (i = R) on a 128-byte
+// boundary and pitch_x is a multiple of 128 bytes, so every wavefront row
+// access of the interior is cache-line aligned.
+#include <hip/hip_runtime.h>
+#include <hip/hip_bf16.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/girih_gpu.h"
+#include "gr_internal.hpp"
+#include "gr_sweep.hpp"
+
+namespace {
+
+thread_local char g_err[512] = "";
+
+int fail(int code, const char *fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                     \
+  do {                                                                                    \
+    hipError_t e_ = (expr);                                                               \
+    if (e_ != hipSuccess)                                                                 \
+      return fail(GR_EHIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, \
+                  __LINE__);                                                              \
+  } while (0)
+
+}  // namespace
+
+namespace gr {
+
+int set_error(int code, const char *msg) { return fail(code, "%s", msg); }
+
+size_t elem_size(int dtype) { return dtype == GR_F64 ? 8 : 4; }
+
+// ------------------------------------------------------------------ launch plumbing
+template <typename T, typename C, int KID, bool FAST, bool ROCS>
+static hipError_t launch_sweep_t(const SweepArgs<T, C> &a0, const LaunchCfg &lc, hipStream_t st) {
+  SweepArgs<T, C> a = a0;
+  constexpr int BY = Tile<T, KID>::BY, RY = Tile<T, KID>::RY;
+  const int TY = BY * RY;
+  const int bx = a.g.xe - a.g.xb, by = a.g.ye - a.g.yb, bz = a.g.ze - a.g.zb;
+  a.tiles_x = (bx + 63) / 64;
+  a.tiles_y = (by + TY - 1) / TY;
+  a.zchunk = lc.zchunk > 0 ? std::min(lc.zchunk, bz) : bz;
+  a.nzc = (bz + a.zchunk - 1) / a.zchunk;
+  const int nb = a.tiles_x * a.tiles_y * a.nzc;
+  hipLaunchKernelGGL((sweep_kernel<T, C, KID, BY, RY, FAST, ROCS>), dim3(nb), dim3(64, BY), 0, st, a);
+  return hipGetLastError();
+}
+
+template <typename T, typename C>
+static hipError_t launch_sweep_k(int kid, bool fast, bool rocs, const SweepArgs<T, C> &a,
+                                 const LaunchCfg &lc, hipStream_t st) {
+  switch (kid) {
+    case 0:
+      if constexpr (std::is_same<T, float>::value) {
+        if (fast) {
+          if (rocs) return launch_sweep_t<T, C, 0, true, true>(a, lc, st);
+          return launch_sweep_t<T, C, 0, true, false>(a, lc, st);
+        }
+      }
+      if (rocs) return launch_sweep_t<T, C, 0, false, true>(a, lc, st);
+      return launch_sweep_t<T, C, 0, false, false>(a, lc, st);
+    case 1: return launch_sweep_t<T, C, 1, false, false>(a, lc, st);
+    case 2: return launch_sweep_t<T, C, 2, false, false>(a, lc, st);
+    case 3: return launch_sweep_t<T, C, 3, false, false>(a, lc, st);
+    case 4: return launch_sweep_t<T, C, 4, false, false>(a, lc, st);
+    case 5: return launch_sweep_t<T, C, 5, false, false>(a, lc, st);
+  }
+  return hipErrorInvalidValue;
+}
+
+// One sweep u <- f(v) with the handle's configuration.
+hipError_t sweep_fields(const Problem &P, void *u, const void *v, const void *src, long long src_idx,
+                        const LaunchCfg &lc, hipStream_t st) {
+  const bool fast = (P.flags & GR_FAST) != 0;
+  const bool rocs = (P.flags & GR_ROC2_SCALAR) != 0;
+  const bool bf16 = (P.flags & GR_COEF_BF16) != 0;
+  Geom g;
+  g.sy = P.pitch;
+  g.sz = (long long)P.pitch * P.rows;
+  g.xb = P.R;
+  g.yb = P.R;
+  g.zb = P.R;
+  g.xe = P.nx + P.R;
+  g.ye = P.ny + P.R;
+  g.ze = P.nz + P.R;
+  if (P.dtype == GR_F64) {
+    SweepArgs<double, double> a{};
+    a.u = (double *)u;
+    a.v = (const double *)v;
+    a.roc2 = (const double *)P.roc2;
+    a.coef = (const double *)P.coef;
+    a.cstride = P.field_elems;
+    for (int r = 0; r < 5; ++r) a.c[r] = P.c[r];
+    a.roc2s = P.roc2s;
+    a.g = g;
+    a.src = (const double *)src;
+    a.src_idx = src_idx;
+    return launch_sweep_k<double, double>(P.kernel, false, rocs, a, lc, st);
+  }
+  if (bf16) {
+    SweepArgs<float, __hip_bfloat16> a{};
+    a.u = (float *)u;
+    a.v = (const float *)v;
+    a.roc2 = (const __hip_bfloat16 *)P.roc2;
+    a.coef = (const __hip_bfloat16 *)P.coef;
+    a.cstride = P.field_elems;
+    for (int r = 0; r < 5; ++r) a.c[r] = (float)P.c[r];
+    a.csum = (float)(P.c[0] + 6.0 * (P.c[1] + P.c[2] + P.c[3] + P.c[4]));
+    a.roc2s = (float)P.roc2s;
+    a.g = g;
+    a.src = (const float *)src;
+    a.src_idx = src_idx;
+    return launch_sweep_k<float, __hip_bfloat16>(P.kernel, fast, rocs, a, lc, st);
+  }
+  SweepArgs<float, float> a{};
+  a.u = (float *)u;
+  a.v = (const float *)v;
+  a.roc2 = (const float *)P.roc2;
+  a.coef = (const float *)P.coef;
+  a.cstride = P.field_elems;
+  for (int r = 0; r < 5; ++r) a.c[r] = (float)P.c[r];
+  a.csum = (float)(P.c[0] + 6.0 * (P.c[1] + P.c[2] + P.c[3] + P.c[4]));
+  a.roc2s = (float)P.roc2s;
+  a.g = g;
+  a.src = (const float *)src;
+  a.src_idx = src_idx;
+  return launch_sweep_k<float, float>(P.kernel, fast, rocs, a, lc, st);
+}
+
+}  // namespace gr
+
+using namespace gr;
+
+struct gr_handle {
+  Problem P;
+  LaunchCfg lc;
+  void *alloc_u1 = nullptr, *alloc_u2 = nullptr, *alloc_roc2 = nullptr, *alloc_coef = nullptr;
+  void *alloc_a = nullptr, *alloc_b = nullptr;
+  void *src = nullptr;  // device source values
+  int nsrc = 0;
+  long long src_idx = -1;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  int parity = 0;      // 0: next application writes U1
+  long long napplied = 0;
+  gr_stats stats{};
+  int device = 0;
+};
+
+static void *field_base(const gr_handle *h, void *alloc) {
+  return alloc ? (char *)alloc + (size_t)h->P.xoff * elem_size(h->P.dtype) : nullptr;
+}
+
+extern "C" const char *gr_last_error(void) { return g_err; }
+
+extern "C" int gr_mwd_round_nt(int nt, int t_dim) {
+  const int w = (t_dim + 1) * 2;
+  const int rem = (nt - 2) % w;
+  return rem != 0 ? nt + w - rem : nt;
+}
+
+extern "C" int gr_create(const gr_desc *d, gr_handle **out) {
+  if (!d || !out) return fail(GR_EINVAL, "gr_create: null argument");
+  *out = nullptr;
+  if (d->kernel < 0 || d->kernel > 5)
+    return fail(GR_EINVAL, "gr_create: kernel %d not supported (regular star kernels 0-5)", d->kernel);
+  if (d->dtype != GR_F64 && d->dtype != GR_F32) return fail(GR_EINVAL, "gr_create: bad dtype %d", d->dtype);
+  if (d->nx < 1 || d->ny < 1 || d->nz < 1) return fail(GR_EINVAL, "gr_create: empty grid");
+  if ((d->flags & GR_FAST) && d->dtype != GR_F32)
+    return fail(GR_EINVAL, "gr_create: GR_FAST applies to GR_F32 only");
+  if ((d->flags & GR_FAST) && d->kernel != 0)
+    return fail(GR_EUNSUPPORTED, "gr_create: GR_FAST is implemented for kernel 0");
+  if ((d->flags & GR_COEF_BF16) && d->dtype != GR_F32)
+    return fail(GR_EINVAL, "gr_create: GR_COEF_BF16 applies to GR_F32 fields");
+  if ((d->flags & GR_ROC2_SCALAR) && d->kernel != 0)
+    return fail(GR_EINVAL, "gr_create: GR_ROC2_SCALAR applies to kernel 0");
+  gr_handle *h = new gr_handle();
+  Problem &P = h->P;
+  P.kernel = d->kernel;
+  P.dtype = d->dtype;
+  P.flags = d->flags;
+  P.nx = d->nx;
+  P.ny = d->ny;
+  P.nz = d->nz;
+  P.R = kradius(d->kernel);
+  P.ncoef = kncoef(d->kernel);
+  for (int r = 0; r < 5; ++r) P.c[r] = d->coef[r];
+  P.roc2s = d->roc2;
+  const int es = (int)elem_size(P.dtype);
+  const int line = 128 / es;                      // elements per 128-byte line
+  P.xoff = (line - P.R % line) % line;            // (xoff + R) % line == 0
+  P.pitch = ((P.xoff + P.nx + 2 * P.R + line - 1) / line) * line;
+  P.rows = P.ny + 2 * P.R;
+  P.planes = P.nz + 2 * P.R;
+  P.field_elems = (long long)P.pitch * P.rows * P.planes;
+  h->lc.zchunk = 0;
+  h->lc.t_block = d->t_block <= 0 ? 1 : d->t_block;
+  const size_t fbytes = (size_t)P.field_elems * es;
+  hipError_t e;
+  int rc = GR_OK;
+  if ((e = hipGetDevice(&h->device)) != hipSuccess) rc = fail(GR_EHIP, "hipGetDevice: %s", hipGetErrorString(e));
+  auto alloc = [&](void **p, size_t n) {
+    if (rc != GR_OK) return;
+    if ((e = hipMalloc(p, n)) != hipSuccess) {
+      rc = fail(GR_ENOMEM, "hipMalloc(%zu) failed: %s", n, hipGetErrorString(e));
+      return;
+    }
+    if ((e = hipMemset(*p, 0, n)) != hipSuccess) rc = fail(GR_EHIP, "hipMemset: %s", hipGetErrorString(e));
+  };
+  alloc(&h->alloc_u1, fbytes);
+  alloc(&h->alloc_u2, fbytes);
+  const size_t cbytes = (P.flags & GR_COEF_BF16) ? (size_t)P.field_elems * 2 : fbytes;
+  if (P.kernel == 0 && !(P.flags & GR_ROC2_SCALAR)) alloc(&h->alloc_roc2, cbytes);
+  if (P.ncoef > 0) alloc(&h->alloc_coef, cbytes * P.ncoef);
+  if (rc == GR_OK && (e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking)) != hipSuccess)
+    rc = fail(GR_EHIP, "hipStreamCreate: %s", hipGetErrorString(e));
+  if (rc == GR_OK && ((e = hipEventCreate(&h->ev0)) != hipSuccess || (e = hipEventCreate(&h->ev1)) != hipSuccess))
+    rc = fail(GR_EHIP, "hipEventCreate: %s", hipGetErrorString(e));
+  if (rc != GR_OK) {
+    gr_destroy(h);
+    return rc;
+  }
+  const size_t ces = (P.flags & GR_COEF_BF16) ? 2 : es;
+  P.roc2 = h->alloc_roc2 ? (char *)h->alloc_roc2 + (size_t)P.xoff * ces : nullptr;
+  P.coef = h->alloc_coef ? (char *)h->alloc_coef + (size_t)P.xoff * ces : nullptr;
+  h->stats.device_bytes = (int64_t)(2 * fbytes + (h->alloc_roc2 ? cbytes : 0) + (size_t)P.ncoef * cbytes);
+  *out = h;
+  return GR_OK;
+}
+
+extern "C" int gr_destroy(gr_handle *h) {
+  if (!h) return GR_OK;
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  for (void *p : {h->alloc_u1, h->alloc_u2, h->alloc_roc2, h->alloc_coef, h->alloc_a, h->alloc_b, h->src})
+    if (p) (void)hipFree(p);
+  if (h->ev0) (void)hipEventDestroy(h->ev0);
+  if (h->ev1) (void)hipEventDestroy(h->ev1);
+  if (h->stream) (void)hipStreamDestroy(h->stream);
+  delete h;
+  return GR_OK;
+}
+
+// host girih layout <-> device layout: both store rows (k*nny + j) at a constant pitch
+static int copy_field(gr_handle *h, const int ld[3], void *dev_base, size_t dev_es, const void *host,
+                      size_t host_es, bool to_dev) {
+  const Problem &P = h->P;
+  const size_t width = (size_t)(P.nx + 2 * P.R);
+  if (dev_es != host_es) return fail(GR_EINVAL, "copy_field: element size mismatch");
+  if (to_dev)
+    HIP_TRY(hipMemcpy2DAsync(dev_base, (size_t)P.pitch * dev_es, host, (size_t)ld[0] * host_es,
+                             width * host_es, (size_t)P.rows * P.planes, hipMemcpyHostToDevice, h->stream));
+  else
+    HIP_TRY(hipMemcpy2DAsync((void *)host, (size_t)ld[0] * host_es, dev_base, (size_t)P.pitch * dev_es,
+                             width * host_es, (size_t)P.rows * P.planes, hipMemcpyDeviceToHost, h->stream));
+  return GR_OK;
+}
+
+static int check_ld(const gr_handle *h, const int ld[3]) {
+  const Problem &P = h->P;
+  if (!ld) return fail(GR_EINVAL, "ldomain_shape is NULL");
+  if (ld[0] < P.nx + 2 * P.R || ld[1] != P.ny + 2 * P.R || ld[2] != P.nz + 2 * P.R)
+    return fail(GR_EINVAL, "ldomain_shape {%d,%d,%d} does not fit grid %dx%dx%d with radius %d", ld[0], ld[1],
+                ld[2], P.nx, P.ny, P.nz, P.R);
+  return GR_OK;
+}
+
+extern "C" int gr_upload(gr_handle *h, const int ld[3], const void *U1, const void *U2, const void *U3,
+                         const void *coef) {
+  if (!h) return fail(GR_EINVAL, "gr_upload: null handle");
+  int rc = check_ld(h, ld);
+  if (rc) return rc;
+  const Problem &P = h->P;
+  const size_t es = elem_size(P.dtype);
+  if (P.flags & GR_COEF_BF16)
+    if ((P.kernel == 0 && U3 && !(P.flags & GR_ROC2_SCALAR)) || (P.ncoef > 0 && coef))
+      return fail(GR_EUNSUPPORTED, "gr_upload: bf16 coefficient fields are filled on the device (gr_init_*)");
+  if (U1 && (rc = copy_field(h, ld, field_base(h, h->alloc_u1), es, U1, es, true))) return rc;
+  if (U2 && (rc = copy_field(h, ld, field_base(h, h->alloc_u2), es, U2, es, true))) return rc;
+  if (U3 && P.roc2 && (rc = copy_field(h, ld, P.roc2, es, U3, es, true))) return rc;
+  if (coef && P.coef) {
+    const size_t hstride = (size_t)ld[0] * ld[1] * ld[2] * es;
+    const size_t dstride = (size_t)P.field_elems * es;
+    for (int m = 0; m < P.ncoef; ++m)
+      if ((rc = copy_field(h, ld, (char *)P.coef + m * dstride, es, (const char *)coef + m * hstride, es, true)))
+        return rc;
+  }
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  h->parity = 0;
+  h->napplied = 0;
+  return GR_OK;
+}
+
+extern "C" int gr_download(gr_handle *h, const int ld[3], void *U1, void *U2) {
+  if (!h) return fail(GR_EINVAL, "gr_download: null handle");
+  int rc = check_ld(h, ld);
+  if (rc) return rc;
+  const size_t es = elem_size(h->P.dtype);
+  if (U1 && (rc = copy_field(h, ld, field_base(h, h->alloc_u1), es, U1, es, false))) return rc;
+  if (U2 && (rc = copy_field(h, ld, field_base(h, h->alloc_u2), es, U2, es, false))) return rc;
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  return GR_OK;
+}
+
+extern "C" int gr_set_source(gr_handle *h, int i, int j, int k, const double *values, int n) {
+  if (!h) return fail(GR_EINVAL, "gr_set_source: null handle");
+  const Problem &P = h->P;
+  if (h->src) {
+    HIP_TRY(hipFree(h->src));
+    h->src = nullptr;
+  }
+  h->nsrc = 0;
+  h->src_idx = -1;
+  if (n <= 0) return GR_OK;
+  if (!values) return fail(GR_EINVAL, "gr_set_source: values is NULL");
+  if (i < P.R || i >= P.nx + P.R || j < P.R || j >= P.ny + P.R || k < P.R || k >= P.nz + P.R)
+    return fail(GR_EINVAL, "gr_set_source: point (%d,%d,%d) is not an interior cell", i, j, k);
+  const size_t es = elem_size(P.dtype);
+  std::vector<char> buf((size_t)n * es);
+  for (int s = 0; s < n; ++s) {
+    if (P.dtype == GR_F64)
+      ((double *)buf.data())[s] = values[s];
+    else
+      ((float *)buf.data())[s] = (float)values[s];
+  }
+  HIP_TRY(hipMalloc(&h->src, buf.size()));
+  HIP_TRY(hipMemcpy(h->src, buf.data(), buf.size(), hipMemcpyHostToDevice));
+  h->nsrc = n;
+  h->src_idx = ((long long)k * P.rows + j) * P.pitch + i;
+  return GR_OK;
+}
+
+extern "C" int gr_step(gr_handle *h, int napply) {
+  if (!h) return fail(GR_EINVAL, "gr_step: null handle");
+  if (napply < 0) return fail(GR_EINVAL, "gr_step: negative step count");
+  const Problem &P = h->P;
+  void *b1 = field_base(h, h->alloc_u1), *b2 = field_base(h, h->alloc_u2);
+  const size_t es = elem_size(P.dtype);
+  HIP_TRY(hipEventRecord(h->ev0, h->stream));
+  for (int s = 0; s < napply; ++s) {
+    const long long app = h->napplied + s;
+    const bool has_src = h->src && app < h->nsrc;
+    const void *srcp = has_src ? (const char *)h->src + (size_t)app * es : nullptr;
+    const long long sidx = has_src ? h->src_idx : -1;
+    hipError_t e = (h->parity == 0) ? sweep_fields(P, b1, b2, srcp, sidx, h->lc, h->stream)
+                                    : sweep_fields(P, b2, b1, srcp, sidx, h->lc, h->stream);
+    if (e != hipSuccess) return fail(GR_EHIP, "sweep launch failed: %s", hipGetErrorString(e));
+    h->parity ^= 1;
+  }
+  HIP_TRY(hipEventRecord(h->ev1, h->stream));
+  HIP_TRY(hipEventSynchronize(h->ev1));
+  float ms = 0.f;
+  HIP_TRY(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+  h->napplied += napply;
+  h->stats.last_ms = ms;
+  h->stats.last_updates = (int64_t)napply * P.nx * P.ny * P.nz;
+  h->stats.last_launches = napply;
+  return GR_OK;
+}
+
+extern "C" int gr_run_ts(gr_handle *h, int ts, int nt, int t_dim, int *nt_applied) {
+  if (!h) return fail(GR_EINVAL, "gr_run_ts: null handle");
+  if (nt < 0) return fail(GR_EINVAL, "gr_run_ts: negative nt");
+  const Problem &P = h->P;
+  if (ts == 0 || ts == 1) {
+    if (nt_applied) *nt_applied = nt;
+    return gr_step(h, 2 * ((nt + 1) / 2));
+  }
+  if (ts != 2) return fail(GR_EINVAL, "gr_run_ts: unknown time stepper %d", ts);
+  // validity rules of the diamond stepper (diamond_utils.c:1023-1098), single rank
+  if (t_dim < 1) return fail(GR_EINVAL, "Diamond method does not support unrolling in time less than 1");
+  if (t_dim % 2 == 0) return fail(GR_EINVAL, "diamond method does not supports even time unrolling");
+  const int dw = P.R * (t_dim + 1) * 2;
+  if (P.ny < dw || P.ny % dw != 0)
+    return fail(GR_EINVAL, "Intra-diamond method requires the sub-domain size to be multiples of the diamond width: %d elements",
+                dw);
+  if (P.nz < t_dim * 2 * P.R + 1)
+    return fail(GR_EINVAL, "The single core wavefront requires a minimum size of %d at the Z direction",
+                t_dim * 2 * P.R + 1);
+  const int ntr = gr_mwd_round_nt(nt, t_dim);
+  if (nt_applied) *nt_applied = ntr;
+  return gr_step(h, ntr - 1);
+}
+
+extern "C" int gr_get_stats(gr_handle *h, gr_stats *out) {
+  if (!h || !out) return fail(GR_EINVAL, "gr_get_stats: null argument");
+  gr_stats s = h->stats;
+  s.t_block = h->lc.t_block;
+  s.tile_x = 64;
+  s.tile_y = 16;
+  s.zchunk = h->lc.zchunk;
+  s.parity = h->parity;
+  s.pitch_x = h->P.pitch;
+  s.rows_y = h->P.rows;
+  s.planes_z = h->P.planes;
+  s.x_offset = h->P.xoff;
+  *out = s;
+  return GR_OK;
+}
+
+extern "C" int gr_field_ptr(gr_handle *h, int which, void **p) {
+  if (!h || !p) return fail(GR_EINVAL, "gr_field_ptr: null argument");
+  switch (which) {
+    case 0: *p = h->alloc_u1; break;
+    case 1: *p = h->alloc_u2; break;
+    case 2: *p = h->alloc_roc2; break;
+    case 3: *p = h->alloc_coef; break;
+    case 4: *p = h->alloc_a; break;
+    case 5: *p = h->alloc_b; break;
+    default: return fail(GR_EINVAL, "gr_field_ptr: bad field %d", which);
+  }
+  return GR_OK;
+}
+
+extern "C" int gr_sweep_dev(int kernel, int dtype, const int shape[3], int xb, int yb, int zb, int xe, int ye,
+                            int ze, const void *coef, void *u, const void *v, const void *roc2) {
+  if (kernel < 0 || kernel > 5) return fail(GR_EINVAL, "gr_sweep_dev: bad kernel %d", kernel);
+  if (dtype != GR_F64 && dtype != GR_F32) return fail(GR_EINVAL, "gr_sweep_dev: bad dtype");
+  if (!shape || !u || !v || !coef) return fail(GR_EINVAL, "gr_sweep_dev: null argument");
+  const int R = kradius(kernel);
+  if (xb < R || yb < R || zb < R || xe > shape[0] - R || ye > shape[1] - R || ze > shape[2] - R || xb >= xe ||
+      yb >= ye || zb >= ze)
+    return fail(GR_EINVAL, "gr_sweep_dev: box must be non-empty and keep %d cells from every face", R);
+  if (kernel == 0 && !roc2) return fail(GR_EINVAL, "gr_sweep_dev: kernel 0 needs roc2");
+  Problem P{};
+  P.kernel = kernel;
+  P.dtype = dtype;
+  P.R = R;
+  P.ncoef = kncoef(kernel);
+  P.pitch = shape[0];
+  P.rows = shape[1];
+  P.field_elems = (long long)shape[0] * shape[1] * shape[2];
+  if (P.ncoef == 0) {
+    for (int r = 0; r <= R; ++r) P.c[r] = dtype == GR_F64 ? ((const double *)coef)[r] : ((const float *)coef)[r];
+  } else {
+    P.coef = (void *)coef;
+  }
+  P.roc2 = (void *)roc2;
+  // express the box through the generic geometry
+  Geom g;
+  g.sy = shape[0];
+  g.sz = (long long)shape[0] * shape[1];
+  g.xb = xb; g.yb = yb; g.zb = zb; g.xe = xe; g.ye = ye; g.ze = ze;
+  LaunchCfg lc;
+  hipError_t e = sweep_box(P, g, u, v, lc, nullptr);
+  if (e != hipSuccess) return fail(GR_EHIP, "gr_sweep_dev: %s", hipGetErrorString(e));
+  HIP_TRY(hipStreamSynchronize(nullptr));
+  return GR_OK;
+}
+
+namespace gr {
+hipError_t sweep_box(const Problem &P, const Geom &g, void *u, const void *v, const LaunchCfg &lc, hipStream_t st) {
+  if (P.dtype == GR_F64) {
+    SweepArgs<double, double> a{};
+    a.u = (double *)u;
+    a.v = (const double *)v;
+    a.roc2 = (const double *)P.roc2;
+    a.coef = (const double *)P.coef;
+    a.cstride = P.field_elems;
+    for (int r = 0; r < 5; ++r) a.c[r] = P.c[r];
+    a.g = g;
+    a.src_idx = -1;
+    return launch_sweep_k<double, double>(P.kernel, false, false, a, lc, st);
+  }
+  SweepArgs<float, float> a{};
+  a.u = (float *)u;
+  a.v = (const float *)v;
+  a.roc2 = (const float *)P.roc2;
+  a.coef = (const float *)P.coef;
+  a.cstride = P.field_elems;
+  for (int r = 0; r < 5; ++r) a.c[r] = (float)P.c[r];
+  a.g = g;
+  a.src_idx = -1;
+  return launch_sweep_k<float, float>(P.kernel, false, false, a, lc, st);
+}
+}  // namespace gr
+
+// ------------------------------------------------------------------ device initial states
+namespace {
+
+__device__ __forceinline__ uint64_t splitmix64(uint64_t x) {
+  x += 0x9E3779B97F4A7C15ull;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+  return x ^ (x >> 31);
+}
+__device__ __forceinline__ double u01(uint64_t seed, uint64_t stream, uint64_t i) {
+  return (double)(splitmix64(seed ^ splitmix64(stream * 0x632BE59BD9B4E019ull + i)) >> 11) * 0x1.0p-53;
+}
+
+// girih's closed-form state (utils.c:605-697, one rank) / uniform / zero
+template <typename T>
+__global__ void init_fields_kernel(T *u1, T *u2, T *r2, int mode, uint64_t seed, int nx, int ny, int nz, int R,
+                                   long long pitch, long long rows, long long total) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= total) return;
+  const int nnx = nx + 2 * R;
+  const long long row = t / nnx;
+  const int i = (int)(t - row * nnx);
+  const int j = (int)(row % rows);
+  const int k = (int)(row / rows);
+  const long long o = row * pitch + i;
+  const bool inner = i >= R && i < nx + R && j >= R && j < ny + R && k >= R && k < nz + R;
+  T val = T(0);
+  if (mode == 0) {
+    if (inner) {
+      const T r = (T)(1.0 / 3 * (1.0 * i / nx + 1.0 * j / ny + 1.0 * k / nz));
+      val = (T)(r * 1.845703);
+    }
+    if (r2) r2[o] = val;
+    if (i == 0 || i == nnx - 1) val = (T)(val + 100.1);
+  } else if (mode == 1) {
+    if (inner) val = (T)u01(seed, 0, (uint64_t)((k - R) * (long long)ny + (j - R)) * nx + (i - R));
+  }
+  u1[o] = val;
+  u2[o] = val;
+}
+
+template <typename C>
+__global__ void init_coef_kernel(C *coef, int ncoef, uint64_t seed, double lo0, double hi0, double lo1, double hi1,
+                                 long long field_elems) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= field_elems) return;
+  for (int m = 0; m < ncoef; ++m) {
+    const double x = u01(seed, 1 + m, (uint64_t)t);
+    const double val = m == 0 ? lo0 + (hi0 - lo0) * x : lo1 + (hi1 - lo1) * x;
+    coef[(long long)m * field_elems + t] = (C)val;
+  }
+}
+
+}  // namespace
+
+extern "C" int gr_init_fields(gr_handle *h, int mode, uint64_t seed) {
+  if (!h) return fail(GR_EINVAL, "gr_init_fields: null handle");
+  if (mode < 0 || mode > 2) return fail(GR_EINVAL, "gr_init_fields: bad mode %d", mode);
+  const Problem &P = h->P;
+  const long long total = (long long)(P.nx + 2 * P.R) * P.rows * P.planes;
+  const int bs = 256;
+  const unsigned nb = (unsigned)((total + bs - 1) / bs);
+  void *b1 = field_base(h, h->alloc_u1), *b2 = field_base(h, h->alloc_u2);
+  if (P.dtype == GR_F64) {
+    hipLaunchKernelGGL(init_fields_kernel<double>, dim3(nb), dim3(bs), 0, h->stream, (double *)b1, (double *)b2,
+                       (P.roc2 && mode == 0) ? (double *)P.roc2 : nullptr, mode, seed, P.nx, P.ny, P.nz, P.R,
+                       (long long)P.pitch, (long long)P.rows, total);
+  } else {
+    if ((P.flags & GR_COEF_BF16) && P.roc2 && mode == 0)
+      return fail(GR_EUNSUPPORTED, "gr_init_fields: closed-form roc2 is not stored as bf16");
+    hipLaunchKernelGGL(init_fields_kernel<float>, dim3(nb), dim3(bs), 0, h->stream, (float *)b1, (float *)b2,
+                       (P.roc2 && mode == 0) ? (float *)P.roc2 : nullptr, mode, seed, P.nx, P.ny, P.nz, P.R,
+                       (long long)P.pitch, (long long)P.rows, total);
+  }
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  h->parity = 0;
+  h->napplied = 0;
+  return GR_OK;
+}
+
+extern "C" int gr_init_coef_uniform(gr_handle *h, uint64_t seed, double lo0, double hi0, double lo1, double hi1) {
+  if (!h) return fail(GR_EINVAL, "gr_init_coef_uniform: null handle");
+  const Problem &P = h->P;
+  if (P.ncoef == 0 || !h->alloc_coef) return fail(GR_EINVAL, "gr_init_coef_uniform: kernel has no coefficient arrays");
+  const int bs = 256;
+  const unsigned nb = (unsigned)((P.field_elems + bs - 1) / bs);
+  if (P.flags & GR_COEF_BF16)
+    hipLaunchKernelGGL(init_coef_kernel<__hip_bfloat16>, dim3(nb), dim3(bs), 0, h->stream,
+                       (__hip_bfloat16 *)h->alloc_coef, P.ncoef, seed, lo0, hi0, lo1, hi1, P.field_elems);
+  else if (P.dtype == GR_F64)
+    hipLaunchKernelGGL(init_coef_kernel<double>, dim3(nb), dim3(bs), 0, h->stream, (double *)h->alloc_coef, P.ncoef,
+                       seed, lo0, hi0, lo1, hi1, P.field_elems);
+  else
+    hipLaunchKernelGGL(init_coef_kernel<float>, dim3(nb), dim3(bs), 0, h->stream, (float *)h->alloc_coef, P.ncoef,
+                       seed, lo0, hi0, lo1, hi1, P.field_elems);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  return GR_OK;
+}
+
+// ------------------------------------------------------------------ FP64 reductions
+namespace {
+
+template <typename TA, typename TB>
+__global__ __launch_bounds__(256) void compare_kernel(const TA *a, long long asy, long long asz, const TB *b,
+                                                      long long bsy, long long bsz, int nx, int ny, int nz, int R,
+                                                      double *part) {
+  // one workgroup per (j, k) row block; grid-stride over rows
+  double sd = 0.0, sr = 0.0, md = 0.0, mr = 0.0;
+  const long long nrows = (long long)ny * nz;
+  for (long long row = blockIdx.x; row < nrows; row += gridDim.x) {
+    const int j = (int)(row % ny) + R, k = (int)(row / ny) + R;
+    for (int i = threadIdx.x; i < nx; i += blockDim.x) {
+      const double x = (double)a[k * asz + j * asy + i + R];
+      const double y = (double)b[k * bsz + j * bsy + i + R];
+      const double d = x - y;
+      sd += d * d;
+      sr += y * y;
+      md = fmax(md, fabs(d));
+      mr = fmax(mr, fabs(y));
+    }
+  }
+  __shared__ double s[4][256];
+  s[0][threadIdx.x] = sd;
+  s[1][threadIdx.x] = sr;
+  s[2][threadIdx.x] = md;
+  s[3][threadIdx.x] = mr;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      s[0][threadIdx.x] += s[0][threadIdx.x + w];
+      s[1][threadIdx.x] += s[1][threadIdx.x + w];
+      s[2][threadIdx.x] = fmax(s[2][threadIdx.x], s[2][threadIdx.x + w]);
+      s[3][threadIdx.x] = fmax(s[3][threadIdx.x], s[3][threadIdx.x + w]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    for (int q = 0; q < 4; ++q) part[blockIdx.x * 4 + q] = s[q][0];
+}
+
+template <typename TA, typename TB>
+hipError_t launch_compare(const gr::Problem &A, const void *a, const gr::Problem &B, const void *b, double *part,
+                          int nblk, hipStream_t st) {
+  hipLaunchKernelGGL((compare_kernel<TA, TB>), dim3(nblk), dim3(256), 0, st, (const TA *)a, (long long)A.pitch,
+                     (long long)A.pitch * A.rows, (const TB *)b, (long long)B.pitch, (long long)B.pitch * B.rows,
+                     A.nx, A.ny, A.nz, A.R, part);
+  return hipGetLastError();
+}
+
+}  // namespace
+
+extern "C" int gr_compare_fields(gr_handle *a, int fa, gr_handle *b, int fb, double out[4]) {
+  if (!a || !b || !out) return fail(GR_EINVAL, "gr_compare_fields: null argument");
+  if (fa < 0 || fa > 1 || fb < 0 || fb > 1) return fail(GR_EINVAL, "gr_compare_fields: field must be 0 (U1) or 1 (U2)");
+  const Problem &A = a->P, &B = b->P;
+  if (A.nx != B.nx || A.ny != B.ny || A.nz != B.nz || A.R != B.R)
+    return fail(GR_EINVAL, "gr_compare_fields: grids differ");
+  const void *pa = field_base(a, fa == 0 ? a->alloc_u1 : a->alloc_u2);
+  const void *pb = field_base(b, fb == 0 ? b->alloc_u1 : b->alloc_u2);
+  const int nblk = 2048;
+  double *part = nullptr;
+  HIP_TRY(hipMalloc(&part, sizeof(double) * 4 * nblk));
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  hipError_t e;
+  if (A.dtype == GR_F64 && B.dtype == GR_F64) e = launch_compare<double, double>(A, pa, B, pb, part, nblk, a->stream);
+  else if (A.dtype == GR_F32 && B.dtype == GR_F64) e = launch_compare<float, double>(A, pa, B, pb, part, nblk, a->stream);
+  else if (A.dtype == GR_F64 && B.dtype == GR_F32) e = launch_compare<double, float>(A, pa, B, pb, part, nblk, a->stream);
+  else e = launch_compare<float, float>(A, pa, B, pb, part, nblk, a->stream);
+  if (e != hipSuccess) {
+    (void)hipFree(part);
+    return fail(GR_EHIP, "compare launch: %s", hipGetErrorString(e));
+  }
+  std::vector<double> h(4 * nblk);
+  e = hipMemcpyAsync(h.data(), part, sizeof(double) * 4 * nblk, hipMemcpyDeviceToHost, a->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(a->stream);
+  (void)hipFree(part);
+  if (e != hipSuccess) return fail(GR_EHIP, "compare copy: %s", hipGetErrorString(e));
+  out[0] = out[1] = out[2] = out[3] = 0.0;
+  for (int i = 0; i < nblk; ++i) {
+    out[0] += h[4 * i];
+    out[1] += h[4 * i + 1];
+    out[2] = std::max(out[2], h[4 * i + 2]);
+    out[3] = std::max(out[3], h[4 * i + 3]);
+  }
+  return GR_OK;
+}

@@ -1,0 +1,314 @@
+// gr_sweep.hpp -- single-sweep star-stencil kernels for gfx950 (CDNA4).
+//
+// One launch applies one time step u <- f(v) over a box of a 3-D field
+// (girih's KERNEL_SIG sweep, src/kernels/stencils_spt_blk.ic:19-48, with the
+// FUNC_BODY operators of src/kernels/stencils.c:27-216).
+//
+// Scheme (2.5-D z streaming):
+//   * a workgroup of 64 x BY lanes owns an output tile of 64 (x) x BY*RY (y)
+//     cells and marches through a chunk of z planes;
+//   * each lane keeps, for its RY cells, a register queue of the 2R+1 planes
+//     k-R .. k+R of v (the z neighbours never touch LDS);
+//   * the current plane k of the tile plus its R-wide xy halo is staged in LDS
+//     (double buffered, one barrier per plane); x and y neighbours come from LDS;
+//   * every global load (next queue plane, next halo ring, u^(n-1), coefficient
+//     arrays) is issued one plane ahead, so HBM latency overlaps the arithmetic;
+//   * x is the 64-lane wavefront axis: every load/store of a row is one
+//     coalesced 256 B (FP32) / 512 B (FP64) transaction;
+//   * workgroups are renumbered so that each XCD processes a contiguous run of
+//     tiles: tiles that share halos run on the same XCD and hit its L2.
+//
+// Arithmetic modes:
+//   * reference order (default): the association and operand order of girih's
+//     FUNC_BODY, compiled with -ffp-contract=off, so FP64 and FP32 results are
+//     bit-identical to girih's serial reference;
+//   * FAST (FP32, kernel 0): per-radius neighbour sums, FMA accumulation and a
+//     TwoSum-compensated 2u^n - u^(n-1) update (only the final store rounds).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <hip/hip_bf16.h>
+#include <stdint.h>
+
+namespace gr {
+
+__host__ __device__ constexpr int kradius(int kid) { return (kid == 0 || kid == 4) ? 4 : 1; }
+__host__ __device__ constexpr int kncoef(int kid) {
+  return kid == 2 ? 2 : kid == 3 ? 4 : kid == 4 ? 13 : kid == 5 ? 7 : 0;
+}
+
+// Field geometry: element (i, j, k) of a field lives at base[k*sz + j*sy + i]
+// where base already includes the x offset of logical x = 0.
+struct Geom {
+  long long sy, sz;
+  int xb, yb, zb, xe, ye, ze;  // output box
+};
+
+template <typename T, typename C>
+struct SweepArgs {
+  T *u;                // output (kernel 0: holds u^(n-1) on entry)
+  const T *v;          // input field
+  const C *roc2;       // kernel 0 roc2 field (unless scalar)
+  const C *coef;       // coefficient arrays, array m at coef + m*cstride
+  long long cstride;
+  T c[5];              // constant coefficients
+  T csum;              // c0 + 6*(c1+..+c4), rounded once from FP64 (FAST kernel 0)
+  T roc2s;             // scalar roc2
+  Geom g;
+  int tiles_x, tiles_y, nzc, zchunk;
+  const T *src;        // point source value (src_idx >= 0)
+  long long src_idx;   // flat index (k*sz + j*sy + i) of the source point, -1 = none
+};
+
+template <typename T, typename C> __device__ __forceinline__ T ld_coef(const C *p) {
+  return static_cast<T>(*p);
+}
+template <> __device__ __forceinline__ float ld_coef<float, __hip_bfloat16>(const __hip_bfloat16 *p) {
+  return __bfloat162float(*p);
+}
+template <> __device__ __forceinline__ double ld_coef<double, __hip_bfloat16>(const __hip_bfloat16 *p) {
+  return static_cast<double>(__bfloat162float(*p));
+}
+
+// Map a linear workgroup id to a logical tile so that each of the 8 XCDs (which
+// receive workgroups round-robin) owns one contiguous range of tiles.
+__device__ __forceinline__ int xcd_remap(int b, int nb) {
+  const int per = nb >> 3, rem = nb & 7;
+  const int xcd = b & 7, loc = b >> 3;
+  return (xcd < rem) ? xcd * (per + 1) + loc : rem * (per + 1) + (xcd - rem) * per + loc;
+}
+
+template <int R, int TY>
+struct Halo {
+  static constexpr int LX = 64 + 2 * R;
+  static constexpr int LY = TY + 2 * R;
+  static constexpr int N = 2 * R * LX + 2 * R * TY;
+  // halo slot c -> (ly, lx) inside the LDS tile
+  __device__ static __forceinline__ void pos(int c, int &ly, int &lx) {
+    if (c < R * LX) {
+      ly = c / LX;
+      lx = c - ly * LX;
+    } else if (c < 2 * R * LX) {
+      const int d = c - R * LX;
+      ly = R + TY + d / LX;
+      lx = d % LX;
+    } else {
+      const int d = c - 2 * R * LX;
+      const int side = d / (R * TY);
+      const int e = d - side * R * TY;
+      ly = R + e / R;
+      lx = (e % R) + side * (64 + R);
+    }
+  }
+};
+
+template <typename T, int KID, bool FAST>
+struct Op;
+
+// ---- kernel 0: 25-point star, second order in time (stencils.c:27-41) ----
+template <typename T>
+struct Op<T, 0, false> {
+  template <typename X, typename Y>
+  __device__ static __forceinline__ T lap(const T *c, T, T v0, const T *q, X xn, Y yn) {
+    // coef[0]*v + coef[r]*(x pair) + coef[r]*(y pair) + coef[r]*(z pair), r = 1..4, left to right
+    T l = c[0] * v0;
+#pragma unroll
+    for (int r = 1; r <= 4; ++r) {
+      l = l + c[r] * (xn(r) + xn(-r));
+      l = l + c[r] * (yn(r) + yn(-r));
+      l = l + c[r] * (q[4 + r] + q[4 - r]);
+    }
+    return l;
+  }
+  __device__ static __forceinline__ T update(T v0, T uo, T roc2, T l) {
+    return (T(2) * v0 - uo) + roc2 * l;
+  }
+};
+
+template <typename T>
+struct Op<T, 0, true> {
+  // Difference form: lap = csum*v + sum_r c_r * sum_axes((a - v) + (b - v)), with
+  // csum = c0 + 6*sum_r c_r taken in FP64 on the host (0 for a consistent Laplacian).
+  // Neighbour pairs are summed before scaling; the differences of a smooth field
+  // are small and nearly exact, which removes the cancellation of c0*v against the
+  // neighbour sums that dominates the FP32 error of the plain form.
+  template <typename X, typename Y>
+  __device__ static __forceinline__ T lap(const T *c, T csum, T v0, const T *q, X xn, Y yn) {
+    T l = csum * v0;
+#pragma unroll
+    for (int r = 4; r >= 1; --r) {
+      const T d = (((xn(r) - v0) + (xn(-r) - v0)) + ((yn(r) - v0) + (yn(-r) - v0))) +
+                  ((q[4 + r] - v0) + (q[4 - r] - v0));
+      l = __builtin_fmaf(c[r], d, l);
+    }
+    return l;
+  }
+  // u^(n+1) = (2u^n - u^(n-1)) + roc2*lap with the subtraction done as an exact
+  // TwoSum; its rounding error joins the small increment before the one final rounding.
+  __device__ static __forceinline__ T update(T v0, T uo, T roc2, T l) {
+    const T a = T(2) * v0, b = -uo;
+    const T s = a + b;
+    const T bb = s - a;
+    const T err = (a - (s - bb)) + (b - bb);
+    return s + __builtin_fmaf(roc2, l, err);
+  }
+};
+
+// Output tile per workgroup: 64 x BY lanes, RY rows per lane (64 x BY*RY cells).
+template <typename T, int KID>
+struct Tile {
+  static constexpr int BY = (KID == 4) ? 8 : 4;
+  static constexpr int RY = (KID == 4) ? 2 : 4;
+};
+
+template <typename T, typename C, int KID, int BY, int RY, bool FAST, bool ROCS>
+__global__ __launch_bounds__(64 * BY) void sweep_kernel(const SweepArgs<T, C> a) {
+  constexpr int R = kradius(KID);
+  constexpr int Q = 2 * R + 1;
+  constexpr int TY = BY * RY;
+  constexpr int NT = 64 * BY;
+  constexpr int NC = kncoef(KID);
+  using H = Halo<R, TY>;
+  constexpr int LX = H::LX, LY = H::LY;
+  constexpr int HS = (H::N + NT - 1) / NT;  // halo slots per lane
+  __shared__ T lds[2][LY][LX];
+
+  const int nb = a.tiles_x * a.tiles_y * a.nzc;
+  const int lb = xcd_remap(blockIdx.x, nb);
+  const int tile_x = lb % a.tiles_x;
+  const int tile_y = (lb / a.tiles_x) % a.tiles_y;
+  const int zc = lb / (a.tiles_x * a.tiles_y);
+
+  const int tx = threadIdx.x, ty = threadIdx.y;
+  const int tid = ty * 64 + tx;
+  const Geom &g = a.g;
+  const int x0 = g.xb + tile_x * 64;
+  const int y0 = g.yb + tile_y * TY;
+  const int z0 = g.zb + zc * a.zchunk;
+  const int z1 = min(z0 + a.zchunk, g.ze);
+
+  const int gx = x0 + tx;
+  const bool xload = gx < g.xe + R;  // needed as a neighbour
+  const bool xout = gx < g.xe;
+  const T *__restrict__ v = a.v;
+  T *__restrict__ u = a.u;
+
+  long long off[RY];
+  bool yload[RY], yout[RY];
+#pragma unroll
+  for (int m = 0; m < RY; ++m) {
+    const int gy = y0 + ty * RY + m;
+    off[m] = (long long)gy * g.sy + gx;
+    yload[m] = xload && (gy < g.ye + R);
+    yout[m] = xout && (gy < g.ye);
+  }
+
+  // halo slots of this lane
+  int hl[HS];
+  long long hoff[HS];
+  bool hok[HS];
+#pragma unroll
+  for (int s = 0; s < HS; ++s) {
+    const int c = tid + s * NT;
+    int ly = 0, lx = 0;
+    if (c < H::N) H::pos(c, ly, lx);
+    const int hx = x0 - R + lx, hy = y0 - R + ly;
+    hl[s] = ly * LX + lx;
+    hoff[s] = (long long)hy * g.sy + hx;
+    hok[s] = (c < H::N) && (hx < g.xe + R) && (hy < g.ye + R);
+  }
+
+  // register queues: planes z0-R .. z0+R
+  T q[RY][Q];
+#pragma unroll
+  for (int m = 0; m < RY; ++m)
+#pragma unroll
+    for (int p = 0; p < Q; ++p)
+      q[m][p] = yload[m] ? v[(long long)(z0 - R + p) * g.sz + off[m]] : T(0);
+  T hv[HS];
+#pragma unroll
+  for (int s = 0; s < HS; ++s) hv[s] = hok[s] ? v[(long long)z0 * g.sz + hoff[s]] : T(0);
+  T c[5];
+#pragma unroll
+  for (int r = 0; r < 5; ++r) c[r] = a.c[r];
+
+  for (int k = z0; k < z1; ++k) {
+    const int buf = (k - z0) & 1;
+    T(*L)[LX] = lds[buf];
+#pragma unroll
+    for (int m = 0; m < RY; ++m) L[R + ty * RY + m][R + tx] = q[m][R];
+#pragma unroll
+    for (int s = 0; s < HS; ++s)
+      if (tid + s * NT < H::N) (&L[0][0])[hl[s]] = hv[s];
+
+    // operands of plane k (arrive while the LDS tile is written and synchronized)
+    T uo[RY], r2[RY];
+    T cf[RY][NC > 0 ? NC : 1];
+#pragma unroll
+    for (int m = 0; m < RY; ++m) {
+      const long long o = (long long)k * g.sz + off[m];
+      if constexpr (KID == 0) {
+        uo[m] = yout[m] ? u[o] : T(0);
+        if constexpr (!ROCS) r2[m] = yout[m] ? ld_coef<T, C>(a.roc2 + o) : T(0);
+      }
+#pragma unroll
+      for (int n = 0; n < NC; ++n) cf[m][n] = yout[m] ? ld_coef<T, C>(a.coef + n * a.cstride + o) : T(0);
+    }
+    // plane k+1 queue entries and halo ring, one plane ahead
+    const bool more = (k + 1) < z1;
+    T nq[RY];
+#pragma unroll
+    for (int m = 0; m < RY; ++m) nq[m] = (more && yload[m]) ? v[(long long)(k + 1 + R) * g.sz + off[m]] : T(0);
+#pragma unroll
+    for (int s = 0; s < HS; ++s) hv[s] = (more && hok[s]) ? v[(long long)(k + 1) * g.sz + hoff[s]] : T(0);
+
+    __syncthreads();
+
+#pragma unroll
+    for (int m = 0; m < RY; ++m) {
+      const int ly = R + ty * RY + m, lx = R + tx;
+      auto xn = [&](int d) { return L[ly][lx + d]; };
+      auto yn = [&](int d) { return L[ly + d][lx]; };
+      const T v0 = q[m][R];
+      T out;
+      if constexpr (KID == 0) {
+        const T roc = ROCS ? a.roc2s : r2[m];
+        out = Op<T, 0, FAST>::update(v0, uo[m], roc, Op<T, 0, FAST>::lap(c, a.csum, v0, q[m], xn, yn));
+      } else if constexpr (KID == 1) {  // stencils.c:73-78
+        out = c[0] * v0 + c[1] * (xn(1) + xn(-1)) + c[1] * (yn(1) + yn(-1)) + c[1] * (q[m][2] + q[m][0]);
+      } else if constexpr (KID == 2) {  // stencils.c:94-99
+        out = cf[m][0] * v0 + cf[m][1] * (xn(1) + xn(-1)) + cf[m][1] * (yn(1) + yn(-1)) +
+              cf[m][1] * (q[m][2] + q[m][0]);
+      } else if constexpr (KID == 3) {  // stencils.c:121-126
+        out = cf[m][0] * v0 + cf[m][1] * (xn(1) + xn(-1)) + cf[m][2] * (yn(1) + yn(-1)) +
+              cf[m][3] * (q[m][2] + q[m][0]);
+      } else if constexpr (KID == 4) {  // stencils.c:148-162
+        T l = cf[m][0] * v0;
+#pragma unroll
+        for (int r = 1; r <= 4; ++r) {
+          l = l + cf[m][3 * r - 2] * (xn(r) + xn(-r));
+          l = l + cf[m][3 * r - 1] * (yn(r) + yn(-r));
+          l = l + cf[m][3 * r] * (q[m][4 + r] + q[m][4 - r]);
+        }
+        out = l;
+      } else {  // KID == 5, stencils.c:193-201
+        out = cf[m][0] * v0 + cf[m][1] * xn(-1) + cf[m][2] * xn(1) + cf[m][3] * yn(-1) +
+              cf[m][4] * yn(1) + cf[m][5] * q[m][0] + cf[m][6] * q[m][2];
+      }
+      if (yout[m]) {
+        const long long o = (long long)k * g.sz + off[m];
+        if (o == a.src_idx) out = out + *a.src;
+        u[o] = out;
+      }
+    }
+
+    // advance queues
+#pragma unroll
+    for (int m = 0; m < RY; ++m) {
+#pragma unroll
+      for (int p = 0; p < Q - 1; ++p) q[m][p] = q[m][p + 1];
+      q[m][Q - 1] = nq[m];
+    }
+  }
+}
+
+}  // namespace gr

@@ -1,0 +1,237 @@
+"""ctypes host binding of libgirih_gpu.so (the C ABI in include/girih_gpu.h).
+
+The binding mirrors girih's stepper interface (src/wrappers.h:28-36,
+src/data_structures.h:236-297): a problem is (kernel, precision, nx, ny, nz),
+host fields use girih's padded layout (utils.c:367-374, x fastest, numpy shape
+(nnz, nny, nnx_padded)), and a run follows girih's step pairing
+(nb_naive_ts.c:187-203).  The library is loaded from girih_amd/lib; there is no
+fallback: if it is missing, every call raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "lib", "libgirih_gpu.so")
+
+GR_OK, GR_EINVAL, GR_EHIP, GR_ENOMEM, GR_EUNSUPPORTED = 0, -1, -2, -3, -4
+GR_F64, GR_F32 = 0, 1
+GR_ROC2_SCALAR, GR_FAST, GR_COEF_BF16 = 0x1, 0x2, 0x4
+
+RADIUS = {0: 4, 1: 1, 2: 1, 3: 1, 4: 4, 5: 1}
+TIME_ORDER = {0: 2, 1: 1, 2: 1, 3: 1, 4: 1, 5: 1}
+NUM_COEF_ARRAYS = {0: 0, 1: 0, 2: 2, 3: 4, 4: 13, 5: 7}
+KERNEL_NAMES = {0: "iso_ref", 1: "iso_ref_2space_1time", 2: "iso_ref_2space_1time_var",
+                3: "iso_ref_2space_1time_var_axsym", 4: "iso_ref_8space_1time_var_axsym",
+                5: "iso_ref_2space_1time_var_nosym"}
+
+# girih's default coefficient list (utils.c:116-119)
+G_COEF = (-0.28472, 0.16000, -0.02000, 0.00254, -0.00018, -0.18472, 0.19, -0.0500, 0.00554, -0.0009, 0.00354)
+
+
+class GirihError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("girih_gpu error %d: %s" % (code, msg))
+        self.code = code
+
+
+class Desc(ctypes.Structure):
+    _fields_ = [("kernel", ctypes.c_int), ("dtype", ctypes.c_int), ("nx", ctypes.c_int), ("ny", ctypes.c_int),
+                ("nz", ctypes.c_int), ("flags", ctypes.c_int), ("t_block", ctypes.c_int),
+                ("reserved", ctypes.c_int), ("roc2", ctypes.c_double), ("coef", ctypes.c_double * 11)]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("last_ms", ctypes.c_double), ("last_updates", ctypes.c_int64), ("last_launches", ctypes.c_int32),
+                ("t_block", ctypes.c_int32), ("tile_x", ctypes.c_int32), ("tile_y", ctypes.c_int32),
+                ("zchunk", ctypes.c_int32), ("parity", ctypes.c_int32), ("device_bytes", ctypes.c_int64),
+                ("pitch_x", ctypes.c_int32), ("rows_y", ctypes.c_int32), ("planes_z", ctypes.c_int32),
+                ("x_offset", ctypes.c_int32)]
+
+
+EXPORTS = ["gr_create", "gr_destroy", "gr_last_error", "gr_upload", "gr_download", "gr_step", "gr_run_ts",
+           "gr_set_source", "gr_get_stats", "gr_mwd_round_nt", "gr_sweep_dev", "gr_field_ptr", "gr_init_fields",
+           "gr_init_coef_uniform", "gr_compare_fields"]
+
+_lib = None
+
+
+def lib():
+    """Load libgirih_gpu.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        # Bind to the HIP runtime torch already ships when torch is present, so the
+        # process holds a single HIP runtime (torch's libamdhip64 has the same soname).
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libgirih_gpu.so not built: run `python -m girih_amd.build` (%s)" % LIB_PATH)
+        L = ctypes.CDLL(LIB_PATH)
+        P, I = ctypes.c_void_p, ctypes.c_int
+        IP = ctypes.POINTER(ctypes.c_int)
+        sig = {
+            "gr_create": ([ctypes.POINTER(Desc), ctypes.POINTER(P)], I),
+            "gr_destroy": ([P], I),
+            "gr_last_error": ([], ctypes.c_char_p),
+            "gr_upload": ([P, IP, P, P, P, P], I),
+            "gr_download": ([P, IP, P, P], I),
+            "gr_step": ([P, I], I),
+            "gr_run_ts": ([P, I, I, I, IP], I),
+            "gr_set_source": ([P, I, I, I, ctypes.POINTER(ctypes.c_double), I], I),
+            "gr_get_stats": ([P, ctypes.POINTER(Stats)], I),
+            "gr_mwd_round_nt": ([I, I], I),
+            "gr_sweep_dev": ([I, I, IP, I, I, I, I, I, I, P, P, P, P], I),
+            "gr_field_ptr": ([P, I, ctypes.POINTER(P)], I),
+            "gr_init_fields": ([P, I, ctypes.c_uint64], I),
+            "gr_init_coef_uniform": ([P, ctypes.c_uint64, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                      ctypes.c_double], I),
+            "gr_compare_fields": ([P, I, P, I, ctypes.POINTER(ctypes.c_double)], I),
+        }
+        for name, (args, res) in sig.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = res
+        _lib = L
+    return _lib
+
+
+def check(rc):
+    if rc != GR_OK:
+        raise GirihError(rc, lib().gr_last_error().decode())
+    return rc
+
+
+def mwd_round_nt(nt, t_dim):
+    return lib().gr_mwd_round_nt(int(nt), int(t_dim))
+
+
+def np_dtype(dtype):
+    return np.float64 if dtype == GR_F64 else np.float32
+
+
+def ldomain_shape(kernel, nx, ny, nz, alignment=8):
+    """girih's padded local domain (utils.c:367-374) as (nnx, nny, nnz)."""
+    R = RADIUS[kernel]
+    nnx = nx + 2 * R
+    pad = (-nnx) % alignment
+    return (nnx + pad, ny + 2 * R, nz + 2 * R)
+
+
+def _ptr(a):
+    if a is None:
+        return None
+    if not a.flags["C_CONTIGUOUS"]:
+        raise ValueError("arrays must be C-contiguous")
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+class Stepper:
+    """One stencil problem resident in HBM (a girih time-stepper target)."""
+
+    def __init__(self, kernel, nx, ny, nz, dtype=GR_F64, flags=0, t_block=1, roc2=0.0, coef=None):
+        self.kernel, self.nx, self.ny, self.nz = kernel, nx, ny, nz
+        self.dtype, self.flags = dtype, flags
+        self.R = RADIUS.get(kernel, 1)
+        d = Desc()
+        d.kernel, d.dtype, d.nx, d.ny, d.nz = kernel, dtype, nx, ny, nz
+        d.flags, d.t_block, d.roc2 = flags, t_block, roc2
+        c = G_COEF if coef is None else coef
+        for i, x in enumerate(list(c)[:11]):
+            d.coef[i] = float(x)
+        self._h = ctypes.c_void_p()
+        check(lib().gr_create(ctypes.byref(d), ctypes.byref(self._h)))
+
+    def close(self):
+        if self._h:
+            lib().gr_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    @property
+    def np_dtype(self):
+        return np_dtype(self.dtype)
+
+    def _ld(self, a):
+        nnz, nny, nnx = a.shape
+        return (ctypes.c_int * 3)(nnx, nny, nnz)
+
+    def upload(self, U1, U2, U3=None, coef=None):
+        dt = self.np_dtype
+        for a in (U1, U2, U3, coef):
+            if a is not None and a.dtype != dt:
+                raise TypeError("expected %s arrays" % np.dtype(dt))
+        check(lib().gr_upload(self._h, self._ld(U1), _ptr(U1), _ptr(U2), _ptr(U3), _ptr(coef)))
+
+    def download(self, shape=None):
+        if shape is None:
+            nnx, nny, nnz = ldomain_shape(self.kernel, self.nx, self.ny, self.nz)
+            shape = (nnz, nny, nnx)
+        U1 = np.zeros(shape, dtype=self.np_dtype)
+        U2 = np.zeros(shape, dtype=self.np_dtype)
+        check(lib().gr_download(self._h, self._ld(U1), _ptr(U1), _ptr(U2)))
+        return U1, U2
+
+    def step(self, napply):
+        check(lib().gr_step(self._h, int(napply)))
+
+    def run_ts(self, target_ts, nt, t_dim=1):
+        out = ctypes.c_int(0)
+        check(lib().gr_run_ts(self._h, int(target_ts), int(nt), int(t_dim), ctypes.byref(out)))
+        return out.value
+
+    def set_source(self, i, j, k, values):
+        vals = np.ascontiguousarray(values, dtype=np.float64)
+        check(lib().gr_set_source(self._h, i, j, k, vals.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                  int(vals.size)))
+
+    def init_fields(self, mode=0, seed=0):
+        check(lib().gr_init_fields(self._h, int(mode), ctypes.c_uint64(seed)))
+
+    def init_coef_uniform(self, seed, lo0, hi0, lo1, hi1):
+        check(lib().gr_init_coef_uniform(self._h, ctypes.c_uint64(seed), lo0, hi0, lo1, hi1))
+
+    def stats(self):
+        s = Stats()
+        check(lib().gr_get_stats(self._h, ctypes.byref(s)))
+        return {f: getattr(s, f) for f, _ in Stats._fields_}
+
+    def compare(self, field, other, other_field):
+        """FP64 device reduction: {sum (a-b)^2, sum b^2, max|a-b|, max|b|} over the interior."""
+        out = (ctypes.c_double * 4)()
+        check(lib().gr_compare_fields(self._h, int(field), other._h, int(other_field), out))
+        return {"sq_diff": out[0], "sq_ref": out[1], "max_diff": out[2], "max_ref": out[3],
+                "rel_l2": (out[0] / out[1]) ** 0.5 if out[1] > 0 else float("nan")}
+
+    def field_ptr(self, which):
+        p = ctypes.c_void_p()
+        check(lib().gr_field_ptr(self._h, int(which), ctypes.byref(p)))
+        return p.value
+
+
+def sweep_dev(kernel, dtype, shape_xyz, box, coef, u_ptr, v_ptr, roc2_ptr=None):
+    """Level-2 entry: one sweep on device arrays in girih layout (KERNEL_SIG).
+
+    coef: numpy constants (kernels 0, 1) or a device pointer (kernels 2-5)."""
+    shape = (ctypes.c_int * 3)(*shape_xyz)
+    xb, yb, zb, xe, ye, ze = box
+    if isinstance(coef, np.ndarray):
+        coef = np.ascontiguousarray(coef, dtype=np_dtype(dtype))
+        cptr = _ptr(coef)
+    else:
+        cptr = ctypes.c_void_p(coef)
+    check(lib().gr_sweep_dev(kernel, dtype, shape, xb, yb, zb, xe, ye, ze, cptr, ctypes.c_void_p(u_ptr),
+                             ctypes.c_void_p(v_ptr), ctypes.c_void_p(roc2_ptr) if roc2_ptr else None))

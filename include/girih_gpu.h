@@ -1,0 +1,154 @@
+/*
+ * girih_gpu.h -- C ABI of the MI355X (gfx950) star-stencil time stepper.
+ *
+ * A drop-in GPU target for girih's time-stepper registry.  girih selects a
+ * stepper as TSList[p->target_ts].func(p) with `void (*func)(Parameters *p)`
+ * (src/wrappers.h:28-36, src/data_structures.h:294-297); the stepper reads
+ * p->U1/U2/U3/coef in the padded layout of src/utils.c:367-374 and leaves the
+ * result in p->U1 and p->U2.  This library is what such a stepper calls: the
+ * host arrays go to HBM once (gr_upload), time steps run on the device
+ * (gr_step), and the fields come back (gr_download).  All entry points return
+ * an int status (GR_OK = 0) and never exit the process; gr_last_error()
+ * describes the last failure of the calling thread.
+ *
+ * Plain C types only; no HIP or torch types cross this boundary.
+ */
+#ifndef GIRIH_GPU_H_
+#define GIRIH_GPU_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GR_ABI_VERSION 1
+
+/* status codes */
+#define GR_OK 0
+#define GR_EINVAL (-1)       /* bad argument (shape, kernel id, box, flags) */
+#define GR_EHIP (-2)         /* a HIP runtime call failed */
+#define GR_ENOMEM (-3)       /* device allocation failed */
+#define GR_EUNSUPPORTED (-4) /* valid request this build does not implement */
+
+/* field precision (girih's real_t: float, or double with -DDP=1, data_structures.h:99-105) */
+#define GR_F64 0
+#define GR_F32 1
+
+/* gr_desc.flags */
+#define GR_ROC2_SCALAR 0x1  /* kernel 0: roc2 is the constant gr_desc.roc2, no U3 array */
+#define GR_FAST 0x2         /* GR_F32 only: neighbour pairs summed per radius before scaling,
+                               FMA accumulation and a compensated (TwoSum) 2u^n-u^(n-1)
+                               update; not bit-identical to girih's operand order */
+#define GR_COEF_BF16 0x4    /* roc2 / coefficient arrays stored in HBM as bfloat16
+                               (computed in the field precision) */
+
+/* Problem descriptor.  Kernel ids are girih's stencil_info_list
+ * (src/kernels/stencils.c:260-269):
+ *   0 25-pt star, 2nd order in time, constant coef (+ roc2 field)  "iso_ref"
+ *   1  7-pt star, 1st order, constant coef                        "iso_ref_2space_1time"
+ *   2  7-pt star, 1st order, variable coef (2 arrays)             "..._var"
+ *   3  7-pt star, 1st order, var axis-symmetric (4 arrays)        "..._var_axsym"
+ *   4 25-pt star, 1st order, var axis-symmetric (13 arrays)       "iso_ref_8space_1time_var_axsym"
+ *   5  7-pt star, 1st order, var no-symmetry (7 arrays)           "..._var_nosym"      */
+typedef struct gr_desc {
+  int kernel;
+  int dtype;        /* GR_F64 / GR_F32 */
+  int nx, ny, nz;   /* interior size (girih --nx --ny --nz) */
+  int flags;        /* GR_ROC2_SCALAR | GR_FAST | GR_COEF_BF16 */
+  int t_block;      /* time steps fused per HBM pass: 0 = autotuned, 1 = single sweep */
+  int reserved;
+  double roc2;      /* constant roc2 when GR_ROC2_SCALAR */
+  double coef[11];  /* constant coefficients, kernels 0 and 1 (girih coef[0..R]) */
+} gr_desc;
+
+typedef struct gr_handle gr_handle;
+
+typedef struct gr_stats {
+  double last_ms;            /* device time of the last gr_step (HIP events) */
+  int64_t last_updates;      /* lattice-site updates of the last gr_step */
+  int32_t last_launches;     /* kernel launches of the last gr_step */
+  int32_t t_block;           /* fused time steps per pass in use */
+  int32_t tile_x, tile_y;    /* output tile of the sweep kernels */
+  int32_t zchunk;            /* z planes streamed per workgroup */
+  int32_t parity;            /* next application writes U1 (0) or U2 (1) */
+  int64_t device_bytes;      /* HBM held by the handle */
+  int32_t pitch_x, rows_y, planes_z; /* device layout of one field */
+  int32_t x_offset;          /* element offset of logical x = 0 within a row */
+} gr_stats;
+
+/* ---- Level 1: time stepper (replaces TSList[ts].func, src/wrappers.h:28-36) ---- */
+
+/* Allocate device fields for the problem (U1, U2, roc2 or coefficient arrays). */
+int gr_create(const gr_desc *desc, gr_handle **out);
+/* Free everything owned by the handle. */
+int gr_destroy(gr_handle *h);
+/* Describe the last error of this thread (static storage). */
+const char *gr_last_error(void);
+
+/* Copy girih host arrays to HBM.  ldomain_shape = {nnx, nny, nnz} of the host arrays
+ * (girih p->ldomain_shape, utils.c:372-374; nnx may include padding); x is fastest.
+ * U3 is roc2 (kernel 0 without GR_ROC2_SCALAR), coef holds girih's SoA coefficient
+ * arrays (coef[m*nnx*nny*nnz + idx], stencils.h:32) for kernels 2-5.  Any pointer
+ * the kernel does not use may be NULL.  Element type follows desc.dtype.
+ * Resets the step parity (the next application writes U1). */
+int gr_upload(gr_handle *h, const int ldomain_shape[3], const void *U1, const void *U2,
+              const void *U3, const void *coef);
+/* Copy U1 and U2 back into girih-layout host arrays (either may be NULL). */
+int gr_download(gr_handle *h, const int ldomain_shape[3], void *U1, void *U2);
+
+/* Apply `napply` single time-step applications, alternating U1 <- f(U2) then
+ * U2 <- f(U1) as girih's steppers do (nb_naive_ts.c:187-203).  girih's spatial
+ * blocking applies 2*ceil(nt/2) applications; its diamond stepper applies
+ * nt-1 after rounding nt (diamond_utils.c:1042-1056).  Synchronous. */
+int gr_step(gr_handle *h, int napply);
+/* girih time-stepper semantics in one call: ts 0/1 -> 2*ceil(nt/2) applications;
+ * ts 2 -> t_dim/nt validated and rounded like the diamond stepper, then nt-1
+ * applications.  *nt_applied (may be NULL) receives the nt girih would report. */
+int gr_run_ts(gr_handle *h, int target_ts, int nt, int t_dim, int *nt_applied);
+
+/* Point source (nb_naive_ts.c:190,196): after application s the value
+ * values[s] is added at halo-inclusive point (i, j, k) of the field just written.
+ * n values are copied to HBM; applications beyond n add nothing.  n = 0 clears. */
+int gr_set_source(gr_handle *h, int i, int j, int k, const double *values, int n);
+
+int gr_get_stats(gr_handle *h, gr_stats *out);
+
+/* nt as girih's diamond stepper rounds it: (nt-2) % (2*(t_dim+1)) == 0. */
+int gr_mwd_round_nt(int nt, int t_dim);
+
+/* ---- Level 2: one sweep on device-resident girih-layout arrays ----
+ * KERNEL_SIG of src/data_structures.h:202-204 with device pointers:
+ * u <- f(v) over the box [xb,xe) x [yb,ye) x [zb,ze) of arrays shaped
+ * shape = {nnx, nny, nnz} (x fastest).  coef: constant list (kernels 0,1, host
+ * pointer to R+1 values) or SoA device arrays (kernels 2-5).  roc2: device array
+ * (kernel 0).  The box must keep R cells from every array face.  Runs on the
+ * default stream and synchronizes. */
+int gr_sweep_dev(int kernel, int dtype, const int shape[3], int xb, int yb, int zb, int xe,
+                 int ye, int ze, const void *coef, void *u, const void *v, const void *roc2);
+
+/* Device pointer of a field for zero-copy interop: which = 0 U1, 1 U2, 2 roc2 / U3,
+ * 3 coefficient arrays, 4 spare field A, 5 spare field B.  Layout via gr_get_stats:
+ * element (i,j,k) at base + x_offset + (k*rows_y + j)*pitch_x + i. */
+int gr_field_ptr(gr_handle *h, int which, void **dev_ptr);
+
+/* Device-side initial states (no host traffic for large grids).
+ * mode 0: girih's closed-form state (utils.c:605-697) on U1, U2 and roc2;
+ * mode 1: interior of U1=U2 uniform [0,1) from `seed`, halos zero;
+ * mode 2: zero fields. */
+int gr_init_fields(gr_handle *h, int mode, uint64_t seed);
+/* Variable coefficients from `seed`: array 0 uniform [lo0,hi0), the others
+ * uniform [lo1,hi1) (kernels 2-5; whole domain incl. halo). */
+int gr_init_coef_uniform(gr_handle *h, uint64_t seed, double lo0, double hi0, double lo1,
+                         double hi1);
+
+/* Global reductions in FP64 over the interior cells: field fa of handle a against
+ * field fb of handle b (fields: 0 U1, 1 U2; same nx, ny, nz and kernel radius; the
+ * precisions may differ).  out = { sum (a-b)^2, sum b^2, max |a-b|, max |b| }.
+ * b may equal a (then fa != fb compares the two fields of one handle). */
+int gr_compare_fields(gr_handle *a, int fa, gr_handle *b, int fb, double out[4]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GIRIH_GPU_H_ */
