@@ -35,6 +35,7 @@ def run_wave(dtype, flags, cfg, steps, warmup):
     from girih_amd import gpu
     st = gpu.Stepper(0, cfg["nx"], cfg["ny"], cfg["nz"], dtype=dtype, flags=flags | gpu.GR_ROC2_SCALAR,
                      roc2=cfg["roc2"], coef=cfg["coef"])
+    tuned = st.autotune(4)  # tile shape x z-chunk search on scratch copies of the fields
     if warmup > 0:
         st.init_fields(2)
         st.step(warmup)
@@ -45,7 +46,9 @@ def run_wave(dtype, flags, cfg, steps, warmup):
     st.step(steps)
     _sync()
     t1 = time.perf_counter()
-    return st, t1 - t0, st.stats()
+    stats = st.stats()
+    stats["tuned_ms_per_step"] = tuned["last_ms"]
+    return st, t1 - t0, stats
 
 
 def cpu_baseline(n, budget_nt):
@@ -143,7 +146,8 @@ def main():
         "data": "synthetic (zero initial field, Ricker point source)",
         "config": {"workload": "25-pt star wave (girih kernel 0), %d^3 + 4-cell halo, %d steps, v=3000 m/s, "
                                "h=10 m, CFL 0.4, 15 Hz Ricker at centre, FP32 fields, compensated update" % (n, K),
-                   "nx": n, "ny": n, "nz": n, "nt": K, "t_block": s32["t_block"]},
+                   "nx": n, "ny": n, "nz": n, "nt": K, "t_block": s32["t_block"],
+                   "tile": [s32["tile_x"], s32["tile_y"]], "zchunk": s32["zchunk"]},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "algo_bytes_per_launch": algo_bytes, "avg_launch_ms": round(launch_ms, 5),

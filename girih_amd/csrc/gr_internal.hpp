@@ -22,7 +22,11 @@ struct Problem {
 struct LaunchCfg {
   int zchunk = 0;   // z planes per workgroup (0 = whole box)
   int t_block = 1;  // fused time steps per pass
+  int variant = 0;  // sweep tile variant (0 = default tile)
 };
+
+int num_tile_variants(int kid, int dtype);
+void tile_of_variant(int kid, int dtype, int variant, int *tx, int *ty);
 
 size_t elem_size(int dtype);
 int set_error(int code, const char *msg);

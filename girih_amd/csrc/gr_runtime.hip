@@ -52,10 +52,9 @@ int set_error(int code, const char *msg) { return fail(code, "%s", msg); }
 size_t elem_size(int dtype) { return dtype == GR_F64 ? 8 : 4; }
 
 // ------------------------------------------------------------------ launch plumbing
-template <typename T, typename C, int KID, bool FAST, bool ROCS>
-static hipError_t launch_sweep_t(const SweepArgs<T, C> &a0, const LaunchCfg &lc, hipStream_t st) {
+template <typename T, typename C, int KID, bool FAST, bool ROCS, int BY, int RY, int MINW>
+static hipError_t launch_cfg(const SweepArgs<T, C> &a0, const LaunchCfg &lc, hipStream_t st) {
   SweepArgs<T, C> a = a0;
-  constexpr int BY = Tile<T, KID>::BY, RY = Tile<T, KID>::RY;
   const int TY = BY * RY;
   const int bx = a.g.xe - a.g.xb, by = a.g.ye - a.g.yb, bz = a.g.ze - a.g.zb;
   a.tiles_x = (bx + 63) / 64;
@@ -63,8 +62,39 @@ static hipError_t launch_sweep_t(const SweepArgs<T, C> &a0, const LaunchCfg &lc,
   a.zchunk = lc.zchunk > 0 ? std::min(lc.zchunk, bz) : bz;
   a.nzc = (bz + a.zchunk - 1) / a.zchunk;
   const int nb = a.tiles_x * a.tiles_y * a.nzc;
-  hipLaunchKernelGGL((sweep_kernel<T, C, KID, BY, RY, FAST, ROCS>), dim3(nb), dim3(64, BY), 0, st, a);
+  hipLaunchKernelGGL((sweep_kernel<T, C, KID, BY, RY, FAST, ROCS, MINW>), dim3(nb), dim3(64, BY), 0, st, a);
   return hipGetLastError();
+}
+
+// Tile variants selectable at run time for the kernels on the hot path (the
+// other kernels use their default tile):
+//   0: 64 x BY lanes x RY rows default   1: 64x32 (8 x 4)   2: 64x16 (8 x 2)
+//   3: 64x32 (8 x 4), >= 4 (FP32) / 3 (FP64) waves per SIMD
+//   4: 64x16 (4 x 4), >= 4 (FP32) / 3 (FP64) waves per SIMD
+int num_tile_variants(int kid, int dtype) {
+  (void)dtype;
+  return (kid == 0 || kid == 1 || kid == 5) ? 5 : 1;
+}
+void tile_of_variant(int kid, int dtype, int variant, int *tx, int *ty) {
+  static const int rows[5] = {16, 32, 16, 32, 16};
+  *tx = 64;
+  *ty = (variant > 0 && variant < num_tile_variants(kid, dtype)) ? rows[variant] : 16;
+}
+
+template <typename T, typename C, int KID, bool FAST, bool ROCS>
+static hipError_t launch_sweep_t(const SweepArgs<T, C> &a, const LaunchCfg &lc, hipStream_t st) {
+  constexpr int BY = Tile<T, KID>::BY, RY = Tile<T, KID>::RY;
+  constexpr int W = std::is_same<T, double>::value ? 3 : 4;
+  if constexpr (KID == 0 || KID == 1 || KID == 5) {
+    switch (lc.variant) {
+      case 1: return launch_cfg<T, C, KID, FAST, ROCS, 8, 4, 0>(a, lc, st);
+      case 2: return launch_cfg<T, C, KID, FAST, ROCS, 8, 2, 0>(a, lc, st);
+      case 3: return launch_cfg<T, C, KID, FAST, ROCS, 8, 4, W>(a, lc, st);
+      case 4: return launch_cfg<T, C, KID, FAST, ROCS, 4, 4, W>(a, lc, st);
+      default: break;
+    }
+  }
+  return launch_cfg<T, C, KID, FAST, ROCS, BY, RY, 0>(a, lc, st);
 }
 
 template <typename T, typename C>
@@ -115,7 +145,8 @@ hipError_t sweep_fields(const Problem &P, void *u, const void *v, const void *sr
     a.roc2s = P.roc2s;
     a.g = g;
     a.src = (const double *)src;
-    a.src_idx = src_idx;
+    a.src_k = src_idx < 0 ? -1 : (int)(src_idx / g.sz);
+    a.src_off = src_idx < 0 ? 0 : (int)(src_idx % g.sz);
     return launch_sweep_k<double, double>(P.kernel, false, rocs, a, lc, st);
   }
   if (bf16) {
@@ -130,7 +161,8 @@ hipError_t sweep_fields(const Problem &P, void *u, const void *v, const void *sr
     a.roc2s = (float)P.roc2s;
     a.g = g;
     a.src = (const float *)src;
-    a.src_idx = src_idx;
+    a.src_k = src_idx < 0 ? -1 : (int)(src_idx / g.sz);
+    a.src_off = src_idx < 0 ? 0 : (int)(src_idx % g.sz);
     return launch_sweep_k<float, __hip_bfloat16>(P.kernel, fast, rocs, a, lc, st);
   }
   SweepArgs<float, float> a{};
@@ -144,7 +176,8 @@ hipError_t sweep_fields(const Problem &P, void *u, const void *v, const void *sr
   a.roc2s = (float)P.roc2s;
   a.g = g;
   a.src = (const float *)src;
-  a.src_idx = src_idx;
+  a.src_k = src_idx < 0 ? -1 : (int)(src_idx / g.sz);
+    a.src_off = src_idx < 0 ? 0 : (int)(src_idx % g.sz);
   return launch_sweep_k<float, float>(P.kernel, fast, rocs, a, lc, st);
 }
 
@@ -406,8 +439,7 @@ extern "C" int gr_get_stats(gr_handle *h, gr_stats *out) {
   if (!h || !out) return fail(GR_EINVAL, "gr_get_stats: null argument");
   gr_stats s = h->stats;
   s.t_block = h->lc.t_block;
-  s.tile_x = 64;
-  s.tile_y = 16;
+  tile_of_variant(h->P.kernel, h->P.dtype, h->lc.variant, &s.tile_x, &s.tile_y);
   s.zchunk = h->lc.zchunk;
   s.parity = h->parity;
   s.pitch_x = h->P.pitch;
@@ -415,6 +447,82 @@ extern "C" int gr_get_stats(gr_handle *h, gr_stats *out) {
   s.planes_z = h->P.planes;
   s.x_offset = h->P.xoff;
   *out = s;
+  return GR_OK;
+}
+
+extern "C" int gr_set_config(gr_handle *h, int variant, int zchunk, int t_block) {
+  if (!h) return fail(GR_EINVAL, "gr_set_config: null handle");
+  if (variant >= 0) {
+    if (variant >= num_tile_variants(h->P.kernel, h->P.dtype))
+      return fail(GR_EINVAL, "gr_set_config: tile variant %d not available for kernel %d", variant, h->P.kernel);
+    h->lc.variant = variant;
+  }
+  if (zchunk >= 0) h->lc.zchunk = zchunk;
+  if (t_block >= 1) {
+    if (t_block > 1) return fail(GR_EUNSUPPORTED, "gr_set_config: fused depth %d not available", t_block);
+    h->lc.t_block = t_block;
+  }
+  return GR_OK;
+}
+
+static int ensure_spares(gr_handle *h) {
+  const size_t fbytes = (size_t)h->P.field_elems * elem_size(h->P.dtype);
+  if (!h->alloc_a) {
+    HIP_TRY(hipMalloc(&h->alloc_a, fbytes));
+    HIP_TRY(hipMemset(h->alloc_a, 0, fbytes));
+  }
+  if (!h->alloc_b) {
+    HIP_TRY(hipMalloc(&h->alloc_b, fbytes));
+    HIP_TRY(hipMemset(h->alloc_b, 0, fbytes));
+  }
+  return GR_OK;
+}
+
+extern "C" int gr_autotune(gr_handle *h, int steps, gr_stats *best) {
+  if (!h) return fail(GR_EINVAL, "gr_autotune: null handle");
+  if (steps < 1) steps = 4;
+  int rc = ensure_spares(h);
+  if (rc) return rc;
+  const Problem &P = h->P;
+  const size_t fbytes = (size_t)P.field_elems * elem_size(P.dtype);
+  HIP_TRY(hipMemcpyAsync(h->alloc_a, h->alloc_u1, fbytes, hipMemcpyDeviceToDevice, h->stream));
+  HIP_TRY(hipMemcpyAsync(h->alloc_b, h->alloc_u2, fbytes, hipMemcpyDeviceToDevice, h->stream));
+  void *a = field_base(h, h->alloc_a), *b = field_base(h, h->alloc_b);
+  const int nz = P.nz;
+  std::vector<int> zchunks = {0};
+  for (int zc : {512, 256, 128, 64, 32})
+    if (zc < nz) zchunks.push_back(zc);
+  LaunchCfg best_lc = h->lc;
+  float best_ms = 1e30f;
+  for (int var = 0; var < num_tile_variants(P.kernel, P.dtype); ++var)
+    for (int zc : zchunks) {
+      LaunchCfg lc = h->lc;
+      lc.variant = var;
+      lc.zchunk = zc;
+      lc.t_block = 1;
+      // one untimed sweep, then `steps` timed sweeps
+      hipError_t e = sweep_fields(P, a, b, nullptr, -1, lc, h->stream);
+      if (e != hipSuccess) return fail(GR_EHIP, "autotune launch: %s", hipGetErrorString(e));
+      HIP_TRY(hipEventRecord(h->ev0, h->stream));
+      for (int s = 0; s < steps; ++s) {
+        e = (s & 1) ? sweep_fields(P, a, b, nullptr, -1, lc, h->stream) : sweep_fields(P, b, a, nullptr, -1, lc, h->stream);
+        if (e != hipSuccess) return fail(GR_EHIP, "autotune launch: %s", hipGetErrorString(e));
+      }
+      HIP_TRY(hipEventRecord(h->ev1, h->stream));
+      HIP_TRY(hipEventSynchronize(h->ev1));
+      float ms = 0.f;
+      HIP_TRY(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+      if (ms < best_ms) {
+        best_ms = ms;
+        best_lc = lc;
+      }
+    }
+  h->lc = best_lc;
+  if (best) {
+    rc = gr_get_stats(h, best);
+    if (rc) return rc;
+    best->last_ms = best_ms / steps;
+  }
   return GR_OK;
 }
 
@@ -479,7 +587,7 @@ hipError_t sweep_box(const Problem &P, const Geom &g, void *u, const void *v, co
     a.cstride = P.field_elems;
     for (int r = 0; r < 5; ++r) a.c[r] = P.c[r];
     a.g = g;
-    a.src_idx = -1;
+    a.src_k = -1;
     return launch_sweep_k<double, double>(P.kernel, false, false, a, lc, st);
   }
   SweepArgs<float, float> a{};
@@ -490,7 +598,7 @@ hipError_t sweep_box(const Problem &P, const Geom &g, void *u, const void *v, co
   a.cstride = P.field_elems;
   for (int r = 0; r < 5; ++r) a.c[r] = (float)P.c[r];
   a.g = g;
-  a.src_idx = -1;
+  a.src_k = -1;
   return launch_sweep_k<float, float>(P.kernel, false, false, a, lc, st);
 }
 }  // namespace gr

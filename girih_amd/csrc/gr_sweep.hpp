@@ -55,8 +55,9 @@ struct SweepArgs {
   T roc2s;             // scalar roc2
   Geom g;
   int tiles_x, tiles_y, nzc, zchunk;
-  const T *src;        // point source value (src_idx >= 0)
-  long long src_idx;   // flat index (k*sz + j*sy + i) of the source point, -1 = none
+  const T *src;        // point source value, added at plane src_k, in-plane offset src_off
+  int src_k;           // -1 = no source
+  int src_off;
 };
 
 template <typename T, typename C> __device__ __forceinline__ T ld_coef(const C *p) {
@@ -160,8 +161,10 @@ struct Tile {
   static constexpr int RY = (KID == 4) ? 2 : 4;
 };
 
-template <typename T, typename C, int KID, int BY, int RY, bool FAST, bool ROCS>
-__global__ __launch_bounds__(64 * BY) void sweep_kernel(const SweepArgs<T, C> a) {
+// MINW > 0 asks the compiler for at least MINW waves per SIMD (caps VGPRs at 512/MINW).
+template <typename T, typename C, int KID, int BY, int RY, bool FAST, bool ROCS, int MINW>
+__global__ __launch_bounds__(64 * BY) __attribute__((amdgpu_waves_per_eu(MINW > 0 ? MINW : 1)))
+void sweep_kernel(const SweepArgs<T, C> a) {
   constexpr int R = kradius(KID);
   constexpr int Q = 2 * R + 1;
   constexpr int TY = BY * RY;
@@ -187,126 +190,139 @@ __global__ __launch_bounds__(64 * BY) void sweep_kernel(const SweepArgs<T, C> a)
   const int z1 = min(z0 + a.zchunk, g.ze);
 
   const int gx = x0 + tx;
-  const bool xload = gx < g.xe + R;  // needed as a neighbour
   const bool xout = gx < g.xe;
-  const T *__restrict__ v = a.v;
-  T *__restrict__ u = a.u;
+  // Every load address is clamped into the part of the field a tile may read
+  // ([xb-R, xe+R) x [yb-R, ye+R)), so loads need no predicate or branch; lanes
+  // outside the output box compute garbage they never store.
+  const int gxc = min(gx, g.xe + R - 1);
 
-  long long off[RY];
-  bool yload[RY], yout[RY];
+  unsigned off[RY];  // in-plane element offsets (fit 32 bits); plane bases are uniform
+  bool yout[RY];
 #pragma unroll
   for (int m = 0; m < RY; ++m) {
     const int gy = y0 + ty * RY + m;
-    off[m] = (long long)gy * g.sy + gx;
-    yload[m] = xload && (gy < g.ye + R);
+    off[m] = (unsigned)(min(gy, g.ye + R - 1) * (int)g.sy + gxc);
     yout[m] = xout && (gy < g.ye);
   }
-
-  // halo slots of this lane
   int hl[HS];
-  long long hoff[HS];
-  bool hok[HS];
+  unsigned hoff[HS];
+  bool hw[HS];
 #pragma unroll
   for (int s = 0; s < HS; ++s) {
     const int c = tid + s * NT;
     int ly = 0, lx = 0;
     if (c < H::N) H::pos(c, ly, lx);
-    const int hx = x0 - R + lx, hy = y0 - R + ly;
+    const int hx = min(x0 - R + lx, g.xe + R - 1), hy = min(y0 - R + ly, g.ye + R - 1);
     hl[s] = ly * LX + lx;
-    hoff[s] = (long long)hy * g.sy + hx;
-    hok[s] = (c < H::N) && (hx < g.xe + R) && (hy < g.ye + R);
+    hoff[s] = (unsigned)(hy * (int)g.sy + hx);
+    hw[s] = c < H::N;
   }
+  auto plane = [&](const T *base, int k) { return base + (long long)k * g.sz; };
+  const int zlast = g.ze + R - 1;  // last plane any load may touch
 
-  // register queues: planes z0-R .. z0+R
+  // z queue of the tile interior: slot i holds plane k-R+((i - p) mod Q) at phase p;
+  // the z loop is unrolled by Q so the rotation is static register renaming.
   T q[RY][Q];
 #pragma unroll
-  for (int m = 0; m < RY; ++m)
+  for (int i = 0; i < Q; ++i) {
+    const T *vp = plane(a.v, z0 - R + i);
 #pragma unroll
-    for (int p = 0; p < Q; ++p)
-      q[m][p] = yload[m] ? v[(long long)(z0 - R + p) * g.sz + off[m]] : T(0);
+    for (int m = 0; m < RY; ++m) q[m][i] = vp[off[m]];
+  }
   T hv[HS];
+  {
+    const T *vp = plane(a.v, z0);
 #pragma unroll
-  for (int s = 0; s < HS; ++s) hv[s] = hok[s] ? v[(long long)z0 * g.sz + hoff[s]] : T(0);
+    for (int s = 0; s < HS; ++s) hv[s] = vp[hoff[s]];
+  }
   T c[5];
 #pragma unroll
   for (int r = 0; r < 5; ++r) c[r] = a.c[r];
 
-  for (int k = z0; k < z1; ++k) {
-    const int buf = (k - z0) & 1;
-    T(*L)[LX] = lds[buf];
+  for (int kb = z0; kb < z1; kb += Q) {
 #pragma unroll
-    for (int m = 0; m < RY; ++m) L[R + ty * RY + m][R + tx] = q[m][R];
+    for (int p = 0; p < Q; ++p) {
+      const int k = kb + p;
+      if (k >= z1) break;
+      const int buf = (k - z0) & 1;
+      T(*L)[LX] = lds[buf];
 #pragma unroll
-    for (int s = 0; s < HS; ++s)
-      if (tid + s * NT < H::N) (&L[0][0])[hl[s]] = hv[s];
+      for (int m = 0; m < RY; ++m) L[R + ty * RY + m][R + tx] = q[m][(p + R) % Q];
+#pragma unroll
+      for (int s = 0; s < HS; ++s)
+        if (hw[s]) (&L[0][0])[hl[s]] = hv[s];
 
-    // operands of plane k (arrive while the LDS tile is written and synchronized)
-    T uo[RY], r2[RY];
-    T cf[RY][NC > 0 ? NC : 1];
+      // operands of plane k (arrive while the LDS tile is written and synchronized)
+      T uo[RY], r2[RY];
+      T cf[RY][NC > 0 ? NC : 1];
+      {
+        const T *up = plane(a.u, k);
+        const long long pk = (long long)k * g.sz;
 #pragma unroll
-    for (int m = 0; m < RY; ++m) {
-      const long long o = (long long)k * g.sz + off[m];
-      if constexpr (KID == 0) {
-        uo[m] = yout[m] ? u[o] : T(0);
-        if constexpr (!ROCS) r2[m] = yout[m] ? ld_coef<T, C>(a.roc2 + o) : T(0);
-      }
+        for (int m = 0; m < RY; ++m) {
+          if constexpr (KID == 0) {
+            uo[m] = up[off[m]];
+            if constexpr (!ROCS) r2[m] = ld_coef<T, C>(a.roc2 + pk + off[m]);
+          }
 #pragma unroll
-      for (int n = 0; n < NC; ++n) cf[m][n] = yout[m] ? ld_coef<T, C>(a.coef + n * a.cstride + o) : T(0);
-    }
-    // plane k+1 queue entries and halo ring, one plane ahead
-    const bool more = (k + 1) < z1;
-    T nq[RY];
-#pragma unroll
-    for (int m = 0; m < RY; ++m) nq[m] = (more && yload[m]) ? v[(long long)(k + 1 + R) * g.sz + off[m]] : T(0);
-#pragma unroll
-    for (int s = 0; s < HS; ++s) hv[s] = (more && hok[s]) ? v[(long long)(k + 1) * g.sz + hoff[s]] : T(0);
-
-    __syncthreads();
-
-#pragma unroll
-    for (int m = 0; m < RY; ++m) {
-      const int ly = R + ty * RY + m, lx = R + tx;
-      auto xn = [&](int d) { return L[ly][lx + d]; };
-      auto yn = [&](int d) { return L[ly + d][lx]; };
-      const T v0 = q[m][R];
-      T out;
-      if constexpr (KID == 0) {
-        const T roc = ROCS ? a.roc2s : r2[m];
-        out = Op<T, 0, FAST>::update(v0, uo[m], roc, Op<T, 0, FAST>::lap(c, a.csum, v0, q[m], xn, yn));
-      } else if constexpr (KID == 1) {  // stencils.c:73-78
-        out = c[0] * v0 + c[1] * (xn(1) + xn(-1)) + c[1] * (yn(1) + yn(-1)) + c[1] * (q[m][2] + q[m][0]);
-      } else if constexpr (KID == 2) {  // stencils.c:94-99
-        out = cf[m][0] * v0 + cf[m][1] * (xn(1) + xn(-1)) + cf[m][1] * (yn(1) + yn(-1)) +
-              cf[m][1] * (q[m][2] + q[m][0]);
-      } else if constexpr (KID == 3) {  // stencils.c:121-126
-        out = cf[m][0] * v0 + cf[m][1] * (xn(1) + xn(-1)) + cf[m][2] * (yn(1) + yn(-1)) +
-              cf[m][3] * (q[m][2] + q[m][0]);
-      } else if constexpr (KID == 4) {  // stencils.c:148-162
-        T l = cf[m][0] * v0;
-#pragma unroll
-        for (int r = 1; r <= 4; ++r) {
-          l = l + cf[m][3 * r - 2] * (xn(r) + xn(-r));
-          l = l + cf[m][3 * r - 1] * (yn(r) + yn(-r));
-          l = l + cf[m][3 * r] * (q[m][4 + r] + q[m][4 - r]);
+          for (int n = 0; n < NC; ++n) cf[m][n] = ld_coef<T, C>(a.coef + n * a.cstride + pk + off[m]);
         }
-        out = l;
-      } else {  // KID == 5, stencils.c:193-201
-        out = cf[m][0] * v0 + cf[m][1] * xn(-1) + cf[m][2] * xn(1) + cf[m][3] * yn(-1) +
-              cf[m][4] * yn(1) + cf[m][5] * q[m][0] + cf[m][6] * q[m][2];
       }
-      if (yout[m]) {
-        const long long o = (long long)k * g.sz + off[m];
-        if (o == a.src_idx) out = out + *a.src;
-        u[o] = out;
+      // plane k+R+1 into the slot freed by plane k-R, halo ring of plane k+1
+      T nq[RY];
+      {
+        const T *vp = plane(a.v, min(k + 1 + R, zlast));
+#pragma unroll
+        for (int m = 0; m < RY; ++m) nq[m] = vp[off[m]];
+        const T *hp = plane(a.v, min(k + 1, zlast));
+#pragma unroll
+        for (int s = 0; s < HS; ++s) hv[s] = hp[hoff[s]];
       }
-    }
 
-    // advance queues
+      __syncthreads();
+
+      T *__restrict__ uk = a.u + (long long)k * g.sz;
+      const bool src_plane = (k == a.src_k);
 #pragma unroll
-    for (int m = 0; m < RY; ++m) {
+      for (int m = 0; m < RY; ++m) {
+        const int ly = R + ty * RY + m, lx = R + tx;
+        auto xn = [&](int d) { return L[ly][lx + d]; };
+        auto yn = [&](int d) { return L[ly + d][lx]; };
+        auto zn = [&](int d) { return q[m][(p + R + d + Q) % Q]; };
+        T zq[Q];
 #pragma unroll
-      for (int p = 0; p < Q - 1; ++p) q[m][p] = q[m][p + 1];
-      q[m][Q - 1] = nq[m];
+        for (int d = -R; d <= R; ++d) zq[R + d] = zn(d);
+        const T v0 = zq[R];
+        T out;
+        if constexpr (KID == 0) {
+          const T roc = ROCS ? a.roc2s : r2[m];
+          out = Op<T, 0, FAST>::update(v0, uo[m], roc, Op<T, 0, FAST>::lap(c, a.csum, v0, zq, xn, yn));
+        } else if constexpr (KID == 1) {  // stencils.c:73-78
+          out = c[0] * v0 + c[1] * (xn(1) + xn(-1)) + c[1] * (yn(1) + yn(-1)) + c[1] * (zq[2] + zq[0]);
+        } else if constexpr (KID == 2) {  // stencils.c:94-99
+          out = cf[m][0] * v0 + cf[m][1] * (xn(1) + xn(-1)) + cf[m][1] * (yn(1) + yn(-1)) +
+                cf[m][1] * (zq[2] + zq[0]);
+        } else if constexpr (KID == 3) {  // stencils.c:121-126
+          out = cf[m][0] * v0 + cf[m][1] * (xn(1) + xn(-1)) + cf[m][2] * (yn(1) + yn(-1)) +
+                cf[m][3] * (zq[2] + zq[0]);
+        } else if constexpr (KID == 4) {  // stencils.c:148-162
+          T l = cf[m][0] * v0;
+#pragma unroll
+          for (int r = 1; r <= 4; ++r) {
+            l = l + cf[m][3 * r - 2] * (xn(r) + xn(-r));
+            l = l + cf[m][3 * r - 1] * (yn(r) + yn(-r));
+            l = l + cf[m][3 * r] * (zq[4 + r] + zq[4 - r]);
+          }
+          out = l;
+        } else {  // KID == 5, stencils.c:193-201
+          out = cf[m][0] * v0 + cf[m][1] * xn(-1) + cf[m][2] * xn(1) + cf[m][3] * yn(-1) +
+                cf[m][4] * yn(1) + cf[m][5] * zq[0] + cf[m][6] * zq[2];
+        }
+        if (src_plane && (int)off[m] == a.src_off) out = out + *a.src;
+        if (yout[m]) uk[off[m]] = out;
+      }
+#pragma unroll
+      for (int m = 0; m < RY; ++m) q[m][p] = nq[m];
     }
   }
 }

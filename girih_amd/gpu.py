@@ -52,7 +52,7 @@ class Stats(ctypes.Structure):
 
 EXPORTS = ["gr_create", "gr_destroy", "gr_last_error", "gr_upload", "gr_download", "gr_step", "gr_run_ts",
            "gr_set_source", "gr_get_stats", "gr_mwd_round_nt", "gr_sweep_dev", "gr_field_ptr", "gr_init_fields",
-           "gr_init_coef_uniform", "gr_compare_fields"]
+           "gr_init_coef_uniform", "gr_compare_fields", "gr_set_config", "gr_autotune"]
 
 _lib = None
 
@@ -89,6 +89,8 @@ def lib():
             "gr_init_coef_uniform": ([P, ctypes.c_uint64, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                       ctypes.c_double], I),
             "gr_compare_fields": ([P, I, P, I, ctypes.POINTER(ctypes.c_double)], I),
+            "gr_set_config": ([P, I, I, I], I),
+            "gr_autotune": ([P, I, ctypes.POINTER(Stats)], I),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -215,6 +217,14 @@ class Stepper:
         check(lib().gr_compare_fields(self._h, int(field), other._h, int(other_field), out))
         return {"sq_diff": out[0], "sq_ref": out[1], "max_diff": out[2], "max_ref": out[3],
                 "rel_l2": (out[0] / out[1]) ** 0.5 if out[1] > 0 else float("nan")}
+
+    def set_config(self, tile_variant=-1, zchunk=-1, t_block=-1):
+        check(lib().gr_set_config(self._h, int(tile_variant), int(zchunk), int(t_block)))
+
+    def autotune(self, steps=4):
+        s = Stats()
+        check(lib().gr_autotune(self._h, int(steps), ctypes.byref(s)))
+        return {f: getattr(s, f) for f, _ in Stats._fields_}
 
     def field_ptr(self, which):
         p = ctypes.c_void_p()

@@ -142,6 +142,17 @@ int gr_init_fields(gr_handle *h, int mode, uint64_t seed);
 int gr_init_coef_uniform(gr_handle *h, uint64_t seed, double lo0, double hi0, double lo1,
                          double hi1);
 
+/* Launch configuration: tile variant of the sweep kernel (0 = default; see
+ * gr_get_stats for the resulting tile), z planes streamed per workgroup (0 = all),
+ * fused time steps per HBM pass (1 = single sweep).  Values < 0 leave a setting
+ * unchanged.  Results never depend on the configuration. */
+int gr_set_config(gr_handle *h, int tile_variant, int zchunk, int t_block);
+/* Autotuner (replaces girih's CPU tile/thread-group search, diamond_utils.c:691-847):
+ * times every tile variant x z-chunk x fused depth candidate for `steps` steps on
+ * scratch copies of the fields, keeps the fastest, and reports it in *best (may be
+ * NULL).  The problem's own fields and step parity are left untouched. */
+int gr_autotune(gr_handle *h, int steps, gr_stats *best);
+
 /* Global reductions in FP64 over the interior cells: field fa of handle a against
  * field fb of handle b (fields: 0 U1, 1 U2; same nx, ny, nz and kernel radius; the
  * precisions may differ).  out = { sum (a-b)^2, sum b^2, max |a-b|, max |b| }.
