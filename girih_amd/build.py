@@ -3,7 +3,8 @@
     python -m girih_amd.build            # libgirih_gpu.so + mwd_kernel_gpu
     python -m girih_amd.build --lib      # library only
 
-Outputs: girih_amd/lib/libgirih_gpu.so, girih_amd/driver/bin/mwd_kernel_gpu.
+Outputs: girih_amd/lib/libgirih_gpu.so and girih_amd/driver/bin/mwd_kernel_gpu_{dp,sp}
+(the FP64 and FP32 builds of the driver, like girih's `make dp` / `make`).
 """
 import argparse
 import os
@@ -17,7 +18,8 @@ CSRC = os.path.join(PKG, "csrc")
 LIBDIR = os.path.join(PKG, "lib")
 LIB = os.path.join(LIBDIR, "libgirih_gpu.so")
 DRIVER_DIR = os.path.join(PKG, "driver")
-DRIVER_BIN = os.path.join(DRIVER_DIR, "bin", "mwd_kernel_gpu")
+DRIVER_BINS = {"dp": os.path.join(DRIVER_DIR, "bin", "mwd_kernel_gpu_dp"),
+               "sp": os.path.join(DRIVER_DIR, "bin", "mwd_kernel_gpu_sp")}
 
 HIP_SOURCES = ["gr_runtime.hip", "gr_tblock.hip"]
 ARCH = os.environ.get("GIRIH_OFFLOAD_ARCH", "gfx950")
@@ -60,12 +62,13 @@ def build_lib(verbose=False):
 
 def build_driver():
     src = os.path.join(DRIVER_DIR, "mwd_kernel_gpu.c")
-    if not os.path.exists(src):
-        return None
-    os.makedirs(os.path.dirname(DRIVER_BIN), exist_ok=True)
-    run(["gcc", "-O2", "-std=c99", "-Wall", "-I" + os.path.join(REPO, "include"), "-o", DRIVER_BIN, src,
-         "-L" + LIBDIR, "-lgirih_gpu", "-Wl,-rpath,$ORIGIN/../../lib", "-lm"])
-    return DRIVER_BIN
+    os.makedirs(os.path.join(DRIVER_DIR, "bin"), exist_ok=True)
+    for prec, out in DRIVER_BINS.items():
+        # -ffp-contract=off: the --verify serial check keeps girih's operand order exactly
+        run(["gcc", "-O2", "-std=c11", "-Wall", "-ffp-contract=off", "-DDP=%d" % (prec == "dp"),
+             "-I" + os.path.join(REPO, "include"), "-o", out, src, "-L" + LIBDIR, "-lgirih_gpu",
+             "-Wl,-rpath,$ORIGIN/../../lib", "-lm"])
+    return DRIVER_BINS
 
 
 def main(argv=None):
