@@ -1,11 +1,11 @@
-"""Known-size copy used to calibrate rocprofv3 FETCH_SIZE / WRITE_SIZE readings.
+"""Known-size streaming kernels used to calibrate rocprofv3 FETCH_SIZE / WRITE_SIZE.
 
     python -m girih_amd.calib [--mib 1024] [--reps 5]
 
-Copies a --mib MiB buffer into another one --reps times (torch's device copy
-kernel).  Each dispatch moves exactly that many bytes in and out of HBM, so the
-counter reading per dispatch divided by the buffer size is the scale factor that
-turns the counters into bytes.
+Runs `dst = src * 1` (torch's elementwise kernel: one plain vector load and one
+store per element) on a --mib MiB buffer --reps times.  Each dispatch reads and
+writes exactly that many bytes, so the counter reading per dispatch divided by
+the buffer size is the factor that turns the counters into bytes.
 """
 import argparse
 
@@ -22,9 +22,9 @@ def main():
     dst = torch.empty_like(src)
     torch.cuda.synchronize()
     for _ in range(a.reps):
-        dst.copy_(src)
+        torch.mul(src, 1.0, out=dst)
     torch.cuda.synchronize()
-    print("copied %d MiB x %d" % (a.mib, a.reps))
+    print("streamed %d MiB x %d" % (a.mib, a.reps))
 
 
 if __name__ == "__main__":

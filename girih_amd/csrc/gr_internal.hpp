@@ -23,6 +23,7 @@ struct LaunchCfg {
   int zchunk = 0;   // z planes per workgroup (0 = whole box)
   int t_block = 1;  // fused time steps per pass
   int variant = 0;  // sweep tile variant (0 = default tile)
+  int xcd_order = 1;  // renumber workgroups per XCD
 };
 
 int num_tile_variants(int kid, int dtype);

@@ -15,6 +15,7 @@
 #include <algorithm>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -52,7 +53,7 @@ int set_error(int code, const char *msg) { return fail(code, "%s", msg); }
 size_t elem_size(int dtype) { return dtype == GR_F64 ? 8 : 4; }
 
 // ------------------------------------------------------------------ launch plumbing
-template <typename T, typename C, int KID, bool FAST, bool ROCS, int BY, int RY, int MINW>
+template <typename T, typename C, int KID, bool FAST, bool ROCS, int BY, int RY, int MINW, bool RING = false>
 static hipError_t launch_cfg(const SweepArgs<T, C> &a0, const LaunchCfg &lc, hipStream_t st) {
   SweepArgs<T, C> a = a0;
   const int TY = BY * RY;
@@ -61,8 +62,9 @@ static hipError_t launch_cfg(const SweepArgs<T, C> &a0, const LaunchCfg &lc, hip
   a.tiles_y = (by + TY - 1) / TY;
   a.zchunk = lc.zchunk > 0 ? std::min(lc.zchunk, bz) : bz;
   a.nzc = (bz + a.zchunk - 1) / a.zchunk;
+  a.xcd_order = lc.xcd_order;
   const int nb = a.tiles_x * a.tiles_y * a.nzc;
-  hipLaunchKernelGGL((sweep_kernel<T, C, KID, BY, RY, FAST, ROCS, MINW>), dim3(nb), dim3(64, BY), 0, st, a);
+  hipLaunchKernelGGL((sweep_kernel<T, C, KID, BY, RY, FAST, ROCS, MINW, RING>), dim3(nb), dim3(64, BY), 0, st, a);
   return hipGetLastError();
 }
 
@@ -71,12 +73,15 @@ static hipError_t launch_cfg(const SweepArgs<T, C> &a0, const LaunchCfg &lc, hip
 //   0: 64 x BY lanes x RY rows default   1: 64x32 (8 x 4)   2: 64x16 (8 x 2)
 //   3: 64x32 (8 x 4), >= 4 (FP32) / 3 (FP64) waves per SIMD
 //   4: 64x16 (4 x 4), >= 4 (FP32) / 3 (FP64) waves per SIMD
+//   5: as 3 with the LDS plane ring (halo fetched with the plane interior)
+//   6: 64x16 (8 x 2) with the LDS plane ring
+//   7: as 4 with the LDS plane ring
 int num_tile_variants(int kid, int dtype) {
   (void)dtype;
-  return (kid == 0 || kid == 1 || kid == 5) ? 5 : 1;
+  return (kid == 0 || kid == 1 || kid == 5) ? 8 : 1;
 }
 void tile_of_variant(int kid, int dtype, int variant, int *tx, int *ty) {
-  static const int rows[5] = {16, 32, 16, 32, 16};
+  static const int rows[8] = {16, 32, 16, 32, 16, 32, 16, 16};
   *tx = 64;
   *ty = (variant > 0 && variant < num_tile_variants(kid, dtype)) ? rows[variant] : 16;
 }
@@ -91,6 +96,9 @@ static hipError_t launch_sweep_t(const SweepArgs<T, C> &a, const LaunchCfg &lc, 
       case 2: return launch_cfg<T, C, KID, FAST, ROCS, 8, 2, 0>(a, lc, st);
       case 3: return launch_cfg<T, C, KID, FAST, ROCS, 8, 4, W>(a, lc, st);
       case 4: return launch_cfg<T, C, KID, FAST, ROCS, 4, 4, W>(a, lc, st);
+      case 5: return launch_cfg<T, C, KID, FAST, ROCS, 8, 4, W, true>(a, lc, st);
+      case 6: return launch_cfg<T, C, KID, FAST, ROCS, 8, 2, 0, true>(a, lc, st);
+      case 7: return launch_cfg<T, C, KID, FAST, ROCS, 4, 4, W, true>(a, lc, st);
       default: break;
     }
   }
@@ -157,7 +165,7 @@ hipError_t sweep_fields(const Problem &P, void *u, const void *v, const void *sr
     a.coef = (const __hip_bfloat16 *)P.coef;
     a.cstride = P.field_elems;
     for (int r = 0; r < 5; ++r) a.c[r] = (float)P.c[r];
-    a.csum = (float)(P.c[0] + 6.0 * (P.c[1] + P.c[2] + P.c[3] + P.c[4]));
+    a.csum = (float)(P.c[0] + 6.0 * (P.c[1] + P.c[2]));
     a.roc2s = (float)P.roc2s;
     a.g = g;
     a.src = (const float *)src;
@@ -172,7 +180,7 @@ hipError_t sweep_fields(const Problem &P, void *u, const void *v, const void *sr
   a.coef = (const float *)P.coef;
   a.cstride = P.field_elems;
   for (int r = 0; r < 5; ++r) a.c[r] = (float)P.c[r];
-  a.csum = (float)(P.c[0] + 6.0 * (P.c[1] + P.c[2] + P.c[3] + P.c[4]));
+  a.csum = (float)(P.c[0] + 6.0 * (P.c[1] + P.c[2]));
   a.roc2s = (float)P.roc2s;
   a.g = g;
   a.src = (const float *)src;
@@ -248,6 +256,10 @@ extern "C" int gr_create(const gr_desc *d, gr_handle **out) {
   P.planes = P.nz + 2 * P.R;
   P.field_elems = (long long)P.pitch * P.rows * P.planes;
   h->lc.zchunk = 0;
+  {  // GIRIH_XCD_ORDER=0 launches tiles in plain order (for measurements)
+    const char *e = getenv("GIRIH_XCD_ORDER");
+    h->lc.xcd_order = (e && e[0] == '0') ? 0 : 1;
+  }
   h->lc.t_block = d->t_block <= 0 ? 1 : d->t_block;
   const size_t fbytes = (size_t)P.field_elems * es;
   hipError_t e;

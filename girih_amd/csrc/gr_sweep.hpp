@@ -51,10 +51,11 @@ struct SweepArgs {
   const C *coef;       // coefficient arrays, array m at coef + m*cstride
   long long cstride;
   T c[5];              // constant coefficients
-  T csum;              // c0 + 6*(c1+..+c4), rounded once from FP64 (FAST kernel 0)
+  T csum;              // FAST kernel 0 centre weight c0 + 6*(c1+c2), rounded once from FP64
   T roc2s;             // scalar roc2
   Geom g;
   int tiles_x, tiles_y, nzc, zchunk;
+  int xcd_order;       // 1: renumber workgroups so each XCD owns a contiguous tile range
   const T *src;        // point source value, added at plane src_k, in-plane offset src_off
   int src_k;           // -1 = no source
   int src_off;
@@ -68,6 +69,18 @@ template <> __device__ __forceinline__ float ld_coef<float, __hip_bfloat16>(cons
 }
 template <> __device__ __forceinline__ double ld_coef<double, __hip_bfloat16>(const __hip_bfloat16 *p) {
   return static_cast<double>(__bfloat162float(*p));
+}
+
+// Loads/stores through a uniform base and a 32-bit byte offset: the compiler then
+// uses the SGPR-base + VGPR-offset global addressing mode (no per-access 64-bit
+// address arithmetic in the vector ALU).
+template <typename X>
+__device__ __forceinline__ X ldb(const void *base, unsigned boff) {
+  return *reinterpret_cast<const X *>(reinterpret_cast<const char *>(base) + boff);
+}
+template <typename X>
+__device__ __forceinline__ void stb(void *base, unsigned boff, X v) {
+  *reinterpret_cast<X *>(reinterpret_cast<char *>(base) + boff) = v;
 }
 
 // Map a linear workgroup id to a logical tile so that each of the 8 XCDs (which
@@ -108,7 +121,7 @@ struct Op;
 // ---- kernel 0: 25-point star, second order in time (stencils.c:27-41) ----
 template <typename T>
 struct Op<T, 0, false> {
-  template <typename X, typename Y>
+  template <int ND, typename X, typename Y>
   __device__ static __forceinline__ T lap(const T *c, T, T v0, const T *q, X xn, Y yn) {
     // coef[0]*v + coef[r]*(x pair) + coef[r]*(y pair) + coef[r]*(z pair), r = 1..4, left to right
     T l = c[0] * v0;
@@ -125,32 +138,38 @@ struct Op<T, 0, false> {
   }
 };
 
+// FAST kernel 0 (FP32).  Radii r <= ND use the difference form
+//   c_r * sum_axes((a - v) + (b - v)),
+// radii r > ND the plain pair sums c_r * sum_axes(a + b); the centre weight is
+// csum = c0 + 6*sum_{r<=ND} c_r, rounded once from FP64 on the host.  The
+// differences of a smooth field are small and nearly exact, which removes the
+// cancellation of c0*v against the large-weight neighbour sums that dominates the
+// FP32 error of the plain form; the small far-radius weights (|c3|, |c4| << |c1|)
+// keep the residual cancellation small at 12 fewer operations than ND = 4.
 template <typename T>
 struct Op<T, 0, true> {
-  // Difference form: lap = csum*v + sum_r c_r * sum_axes((a - v) + (b - v)), with
-  // csum = c0 + 6*sum_r c_r taken in FP64 on the host (0 for a consistent Laplacian).
-  // Neighbour pairs are summed before scaling; the differences of a smooth field
-  // are small and nearly exact, which removes the cancellation of c0*v against the
-  // neighbour sums that dominates the FP32 error of the plain form.
-  template <typename X, typename Y>
+  template <int ND, typename X, typename Y>
   __device__ static __forceinline__ T lap(const T *c, T csum, T v0, const T *q, X xn, Y yn) {
     T l = csum * v0;
 #pragma unroll
     for (int r = 4; r >= 1; --r) {
-      const T d = (((xn(r) - v0) + (xn(-r) - v0)) + ((yn(r) - v0) + (yn(-r) - v0))) +
-                  ((q[4 + r] - v0) + (q[4 - r] - v0));
+      T d;
+      if (r <= ND)
+        d = (((xn(r) - v0) + (xn(-r) - v0)) + ((yn(r) - v0) + (yn(-r) - v0))) +
+            ((q[4 + r] - v0) + (q[4 - r] - v0));
+      else
+        d = ((xn(r) + xn(-r)) + (yn(r) + yn(-r))) + (q[4 + r] + q[4 - r]);
       l = __builtin_fmaf(c[r], d, l);
     }
     return l;
   }
-  // u^(n+1) = (2u^n - u^(n-1)) + roc2*lap with the subtraction done as an exact
-  // TwoSum; its rounding error joins the small increment before the one final rounding.
+  // u^(n+1) = u^n + ((u^n - u^(n-1)) + roc2*lap): the increment form of the leapfrog.
+  // u^n - u^(n-1) of a smoothly evolving field is exact (Sterbenz), the small
+  // increment takes the Laplacian term with one rounding, and only the final add
+  // rounds at the magnitude of u -- instead of 2u^n - u^(n-1) losing the low bits
+  // of the increment.
   __device__ static __forceinline__ T update(T v0, T uo, T roc2, T l) {
-    const T a = T(2) * v0, b = -uo;
-    const T s = a + b;
-    const T bb = s - a;
-    const T err = (a - (s - bb)) + (b - bb);
-    return s + __builtin_fmaf(roc2, l, err);
+    return v0 + __builtin_fmaf(roc2, l, v0 - uo);
   }
 };
 
@@ -162,7 +181,11 @@ struct Tile {
 };
 
 // MINW > 0 asks the compiler for at least MINW waves per SIMD (caps VGPRs at 512/MINW).
-template <typename T, typename C, int KID, int BY, int RY, bool FAST, bool ROCS, int MINW>
+// RING: the LDS holds a ring of R+2 tile planes and the halo ring of a plane is
+// fetched in the same step as the interior of that plane (R+1 planes ahead), when
+// the neighbouring tiles fetch it too and it is still in L2; otherwise the LDS is
+// double buffered and the halo is fetched one plane ahead.
+template <typename T, typename C, int KID, int BY, int RY, bool FAST, bool ROCS, int MINW, bool RING, int ND = 2>
 __global__ __launch_bounds__(64 * BY) __attribute__((amdgpu_waves_per_eu(MINW > 0 ? MINW : 1)))
 void sweep_kernel(const SweepArgs<T, C> a) {
   constexpr int R = kradius(KID);
@@ -173,10 +196,11 @@ void sweep_kernel(const SweepArgs<T, C> a) {
   using H = Halo<R, TY>;
   constexpr int LX = H::LX, LY = H::LY;
   constexpr int HS = (H::N + NT - 1) / NT;  // halo slots per lane
-  __shared__ T lds[2][LY][LX];
+  constexpr int S = RING ? R + 2 : 2;        // LDS tile planes
+  __shared__ T lds[S][LY][LX];
 
   const int nb = a.tiles_x * a.tiles_y * a.nzc;
-  const int lb = xcd_remap(blockIdx.x, nb);
+  const int lb = a.xcd_order ? xcd_remap(blockIdx.x, nb) : (int)blockIdx.x;
   const int tile_x = lb % a.tiles_x;
   const int tile_y = (lb / a.tiles_x) % a.tiles_y;
   const int zc = lb / (a.tiles_x * a.tiles_y);
@@ -201,7 +225,7 @@ void sweep_kernel(const SweepArgs<T, C> a) {
 #pragma unroll
   for (int m = 0; m < RY; ++m) {
     const int gy = y0 + ty * RY + m;
-    off[m] = (unsigned)(min(gy, g.ye + R - 1) * (int)g.sy + gxc);
+    off[m] = (unsigned)(min(gy, g.ye + R - 1) * (int)g.sy + gxc) * (unsigned)sizeof(T);  // bytes
     yout[m] = xout && (gy < g.ye);
   }
   int hl[HS];
@@ -214,7 +238,7 @@ void sweep_kernel(const SweepArgs<T, C> a) {
     if (c < H::N) H::pos(c, ly, lx);
     const int hx = min(x0 - R + lx, g.xe + R - 1), hy = min(y0 - R + ly, g.ye + R - 1);
     hl[s] = ly * LX + lx;
-    hoff[s] = (unsigned)(hy * (int)g.sy + hx);
+    hoff[s] = (unsigned)(hy * (int)g.sy + hx) * (unsigned)sizeof(T);  // bytes
     hw[s] = c < H::N;
   }
   auto plane = [&](const T *base, int k) { return base + (long long)k * g.sz; };
@@ -227,13 +251,22 @@ void sweep_kernel(const SweepArgs<T, C> a) {
   for (int i = 0; i < Q; ++i) {
     const T *vp = plane(a.v, z0 - R + i);
 #pragma unroll
-    for (int m = 0; m < RY; ++m) q[m][i] = vp[off[m]];
+    for (int m = 0; m < RY; ++m) q[m][i] = ldb<T>(vp, off[m]);
   }
   T hv[HS];
-  {
-    const T *vp = plane(a.v, z0);
+  if constexpr (RING) {
+    for (int j = 0; j < R; ++j) {  // halo rings of planes z0 .. z0+R-1 straight into their slots
+      const T *vp = plane(a.v, z0 + j);
+      T *Ls = &lds[(z0 + j) % S][0][0];
 #pragma unroll
-    for (int s = 0; s < HS; ++s) hv[s] = vp[hoff[s]];
+      for (int s = 0; s < HS; ++s)
+        if (hw[s]) Ls[hl[s]] = ldb<T>(vp, hoff[s]);
+    }
+  }
+  {
+    const T *vp = plane(a.v, min(RING ? z0 + R : z0, zlast));
+#pragma unroll
+    for (int s = 0; s < HS; ++s) hv[s] = ldb<T>(vp, hoff[s]);
   }
   T c[5];
 #pragma unroll
@@ -244,13 +277,16 @@ void sweep_kernel(const SweepArgs<T, C> a) {
     for (int p = 0; p < Q; ++p) {
       const int k = kb + p;
       if (k >= z1) break;
-      const int buf = (k - z0) & 1;
+      const int buf = RING ? k % S : (k - z0) & 1;
       T(*L)[LX] = lds[buf];
 #pragma unroll
       for (int m = 0; m < RY; ++m) L[R + ty * RY + m][R + tx] = q[m][(p + R) % Q];
+      {
+        T *Lh = RING ? &lds[(k + R) % S][0][0] : &L[0][0];  // halo of plane k+R (ring) or k
 #pragma unroll
-      for (int s = 0; s < HS; ++s)
-        if (hw[s]) (&L[0][0])[hl[s]] = hv[s];
+        for (int s = 0; s < HS; ++s)
+          if (hw[s]) Lh[hl[s]] = hv[s];
+      }
 
       // operands of plane k (arrive while the LDS tile is written and synchronized)
       T uo[RY], r2[RY];
@@ -261,11 +297,12 @@ void sweep_kernel(const SweepArgs<T, C> a) {
 #pragma unroll
         for (int m = 0; m < RY; ++m) {
           if constexpr (KID == 0) {
-            uo[m] = up[off[m]];
-            if constexpr (!ROCS) r2[m] = ld_coef<T, C>(a.roc2 + pk + off[m]);
+            uo[m] = ldb<T>(up, off[m]);
+            if constexpr (!ROCS) r2[m] = ld_coef<T, C>(a.roc2 + pk + off[m] / (unsigned)sizeof(T));
           }
 #pragma unroll
-          for (int n = 0; n < NC; ++n) cf[m][n] = ld_coef<T, C>(a.coef + n * a.cstride + pk + off[m]);
+          for (int n = 0; n < NC; ++n)
+            cf[m][n] = ld_coef<T, C>(a.coef + n * a.cstride + pk + off[m] / (unsigned)sizeof(T));
         }
       }
       // plane k+R+1 into the slot freed by plane k-R, halo ring of plane k+1
@@ -273,10 +310,10 @@ void sweep_kernel(const SweepArgs<T, C> a) {
       {
         const T *vp = plane(a.v, min(k + 1 + R, zlast));
 #pragma unroll
-        for (int m = 0; m < RY; ++m) nq[m] = vp[off[m]];
-        const T *hp = plane(a.v, min(k + 1, zlast));
+        for (int m = 0; m < RY; ++m) nq[m] = ldb<T>(vp, off[m]);
+        const T *hp = plane(a.v, min(RING ? k + 1 + R : k + 1, zlast));
 #pragma unroll
-        for (int s = 0; s < HS; ++s) hv[s] = hp[hoff[s]];
+        for (int s = 0; s < HS; ++s) hv[s] = ldb<T>(hp, hoff[s]);
       }
 
       __syncthreads();
@@ -296,7 +333,7 @@ void sweep_kernel(const SweepArgs<T, C> a) {
         T out;
         if constexpr (KID == 0) {
           const T roc = ROCS ? a.roc2s : r2[m];
-          out = Op<T, 0, FAST>::update(v0, uo[m], roc, Op<T, 0, FAST>::lap(c, a.csum, v0, zq, xn, yn));
+          out = Op<T, 0, FAST>::update(v0, uo[m], roc, Op<T, 0, FAST>::template lap<ND>(c, a.csum, v0, zq, xn, yn));
         } else if constexpr (KID == 1) {  // stencils.c:73-78
           out = c[0] * v0 + c[1] * (xn(1) + xn(-1)) + c[1] * (yn(1) + yn(-1)) + c[1] * (zq[2] + zq[0]);
         } else if constexpr (KID == 2) {  // stencils.c:94-99
@@ -318,8 +355,8 @@ void sweep_kernel(const SweepArgs<T, C> a) {
           out = cf[m][0] * v0 + cf[m][1] * xn(-1) + cf[m][2] * xn(1) + cf[m][3] * yn(-1) +
                 cf[m][4] * yn(1) + cf[m][5] * zq[0] + cf[m][6] * zq[2];
         }
-        if (src_plane && (int)off[m] == a.src_off) out = out + *a.src;
-        if (yout[m]) uk[off[m]] = out;
+        if (src_plane && off[m] == (unsigned)a.src_off * (unsigned)sizeof(T)) out = out + *a.src;
+        if (yout[m]) stb<T>(uk, off[m], out);
       }
 #pragma unroll
       for (int m = 0; m < RY; ++m) q[m][p] = nq[m];
