@@ -256,9 +256,9 @@ extern "C" int gr_create(const gr_desc *d, gr_handle **out) {
   P.planes = P.nz + 2 * P.R;
   P.field_elems = (long long)P.pitch * P.rows * P.planes;
   h->lc.zchunk = 0;
-  {  // GIRIH_XCD_ORDER=0 launches tiles in plain order (for measurements)
+  {  // GIRIH_XCD_ORDER=1 starts from XCD-contiguous tile order (the autotuner tries both)
     const char *e = getenv("GIRIH_XCD_ORDER");
-    h->lc.xcd_order = (e && e[0] == '0') ? 0 : 1;
+    h->lc.xcd_order = (e && e[0] == '1') ? 1 : 0;
   }
   h->lc.t_block = d->t_block <= 0 ? 1 : d->t_block;
   const size_t fbytes = (size_t)P.field_elems * es;
@@ -506,29 +506,45 @@ extern "C" int gr_autotune(gr_handle *h, int steps, gr_stats *best) {
     if (zc < nz) zchunks.push_back(zc);
   LaunchCfg best_lc = h->lc;
   float best_ms = 1e30f;
-  for (int var = 0; var < num_tile_variants(P.kernel, P.dtype); ++var)
-    for (int zc : zchunks) {
-      LaunchCfg lc = h->lc;
-      lc.variant = var;
-      lc.zchunk = zc;
-      lc.t_block = 1;
-      // one untimed sweep, then `steps` timed sweeps
-      hipError_t e = sweep_fields(P, a, b, nullptr, -1, lc, h->stream);
+  auto trial = [&](const LaunchCfg &lc) -> int {
+    // one untimed sweep, then `steps` timed sweeps
+    hipError_t e = sweep_fields(P, a, b, nullptr, -1, lc, h->stream);
+    if (e != hipSuccess) return fail(GR_EHIP, "autotune launch: %s", hipGetErrorString(e));
+    HIP_TRY(hipEventRecord(h->ev0, h->stream));
+    for (int s = 0; s < steps; ++s) {
+      e = (s & 1) ? sweep_fields(P, a, b, nullptr, -1, lc, h->stream) : sweep_fields(P, b, a, nullptr, -1, lc, h->stream);
       if (e != hipSuccess) return fail(GR_EHIP, "autotune launch: %s", hipGetErrorString(e));
-      HIP_TRY(hipEventRecord(h->ev0, h->stream));
-      for (int s = 0; s < steps; ++s) {
-        e = (s & 1) ? sweep_fields(P, a, b, nullptr, -1, lc, h->stream) : sweep_fields(P, b, a, nullptr, -1, lc, h->stream);
-        if (e != hipSuccess) return fail(GR_EHIP, "autotune launch: %s", hipGetErrorString(e));
-      }
-      HIP_TRY(hipEventRecord(h->ev1, h->stream));
-      HIP_TRY(hipEventSynchronize(h->ev1));
-      float ms = 0.f;
-      HIP_TRY(hipEventElapsedTime(&ms, h->ev0, h->ev1));
-      if (ms < best_ms) {
-        best_ms = ms;
-        best_lc = lc;
-      }
     }
+    HIP_TRY(hipEventRecord(h->ev1, h->stream));
+    HIP_TRY(hipEventSynchronize(h->ev1));
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+    if (ms < best_ms) {
+      best_ms = ms;
+      best_lc = lc;
+    }
+    return GR_OK;
+  };
+  // stage 1: tile variant x tile order at each z-chunk size of a coarse set;
+  // stage 2: all z-chunk sizes for the best variant and order
+  for (int var = 0; var < num_tile_variants(P.kernel, P.dtype); ++var)
+    for (int order = 0; order < 2; ++order)
+      for (int zc : {0, std::min(128, nz)}) {
+        LaunchCfg lc = h->lc;
+        lc.variant = var;
+        lc.xcd_order = order;
+        lc.zchunk = zc;
+        lc.t_block = 1;
+        if ((rc = trial(lc))) return rc;
+      }
+  {
+    const LaunchCfg base = best_lc;
+    for (int zc : zchunks) {
+      LaunchCfg lc = base;
+      lc.zchunk = zc;
+      if ((rc = trial(lc))) return rc;
+    }
+  }
   h->lc = best_lc;
   if (best) {
     rc = gr_get_stats(h, best);
