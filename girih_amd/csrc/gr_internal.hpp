@@ -33,6 +33,9 @@ size_t elem_size(int dtype);
 int set_error(int code, const char *msg);
 hipError_t sweep_fields(const Problem &P, void *u, const void *v, const void *src, long long src_idx,
                         const LaunchCfg &lc, hipStream_t st);
+hipError_t tblock_pass(const Problem &P, int tl, const void *X, const void *Y, void *ZO, void *ZN, const void *src,
+                       long long src_idx, const LaunchCfg &lc, hipStream_t st);
+int tb_depths(int kid, int dtype, int *out, int max);
 hipError_t sweep_box(const Problem &P, const Geom &g, void *u, const void *v, const LaunchCfg &lc, hipStream_t st);
 
 }  // namespace gr

@@ -27,6 +27,7 @@
 namespace {
 
 thread_local char g_err[512] = "";
+constexpr int kSrcPad = 64;  // zero source values past the end (fused passes read up to 64 ahead)
 
 int fail(int code, const char *fmt, ...) {
   va_list ap;
@@ -196,9 +197,15 @@ using namespace gr;
 struct gr_handle {
   Problem P;
   LaunchCfg lc;
-  void *alloc_u1 = nullptr, *alloc_u2 = nullptr, *alloc_roc2 = nullptr, *alloc_coef = nullptr;
-  void *alloc_a = nullptr, *alloc_b = nullptr;
-  void *src = nullptr;  // device source values
+  // Physical field buffers: 0 and 1 always exist; 2 and 3 are spares for fused
+  // kernel-0 passes and for the autotuner.  lu[] maps logical U1 / U2 onto them.
+  void *fbuf[4] = {nullptr, nullptr, nullptr, nullptr};
+  int lu[2] = {0, 1};
+  bool spares_synced = false;  // spares hold the same halo as U1/U2
+  int halo_equal = -1;         // U1 and U2 hold the same halo (-1: not yet checked)
+  void *alloc_roc2 = nullptr, *alloc_coef = nullptr;
+  void *flag = nullptr;  // one device int for checks
+  void *src = nullptr;  // device source values (padded with zeros)
   int nsrc = 0;
   long long src_idx = -1;
   hipStream_t stream = nullptr;
@@ -211,6 +218,20 @@ struct gr_handle {
 
 static void *field_base(const gr_handle *h, void *alloc) {
   return alloc ? (char *)alloc + (size_t)h->P.xoff * elem_size(h->P.dtype) : nullptr;
+}
+// base of logical field l (0 = U1, 1 = U2)
+static void *ubase(const gr_handle *h, int l) { return field_base(h, h->fbuf[h->lu[l]]); }
+static void spare_ids(const gr_handle *h, int out[2]) {
+  int n = 0;
+  for (int i = 0; i < 4 && n < 2; ++i)
+    if (i != h->lu[0] && i != h->lu[1]) out[n++] = i;
+}
+// the fields were (re)written from outside: parity restarts, spares and halo facts are stale
+static void state_reset(gr_handle *h) {
+  h->parity = 0;
+  h->napplied = 0;
+  h->spares_synced = false;
+  h->halo_equal = -1;
 }
 
 extern "C" const char *gr_last_error(void) { return g_err; }
@@ -273,8 +294,8 @@ extern "C" int gr_create(const gr_desc *d, gr_handle **out) {
     }
     if ((e = hipMemset(*p, 0, n)) != hipSuccess) rc = fail(GR_EHIP, "hipMemset: %s", hipGetErrorString(e));
   };
-  alloc(&h->alloc_u1, fbytes);
-  alloc(&h->alloc_u2, fbytes);
+  alloc(&h->fbuf[0], fbytes);
+  alloc(&h->fbuf[1], fbytes);
   const size_t cbytes = (P.flags & GR_COEF_BF16) ? (size_t)P.field_elems * 2 : fbytes;
   if (P.kernel == 0 && !(P.flags & GR_ROC2_SCALAR)) alloc(&h->alloc_roc2, cbytes);
   if (P.ncoef > 0) alloc(&h->alloc_coef, cbytes * P.ncoef);
@@ -297,7 +318,7 @@ extern "C" int gr_create(const gr_desc *d, gr_handle **out) {
 extern "C" int gr_destroy(gr_handle *h) {
   if (!h) return GR_OK;
   if (h->stream) (void)hipStreamSynchronize(h->stream);
-  for (void *p : {h->alloc_u1, h->alloc_u2, h->alloc_roc2, h->alloc_coef, h->alloc_a, h->alloc_b, h->src})
+  for (void *p : {h->fbuf[0], h->fbuf[1], h->fbuf[2], h->fbuf[3], h->alloc_roc2, h->alloc_coef, h->src, h->flag})
     if (p) (void)hipFree(p);
   if (h->ev0) (void)hipEventDestroy(h->ev0);
   if (h->ev1) (void)hipEventDestroy(h->ev1);
@@ -340,8 +361,8 @@ extern "C" int gr_upload(gr_handle *h, const int ld[3], const void *U1, const vo
   if (P.flags & GR_COEF_BF16)
     if ((P.kernel == 0 && U3 && !(P.flags & GR_ROC2_SCALAR)) || (P.ncoef > 0 && coef))
       return fail(GR_EUNSUPPORTED, "gr_upload: bf16 coefficient fields are filled on the device (gr_init_*)");
-  if (U1 && (rc = copy_field(h, ld, field_base(h, h->alloc_u1), es, U1, es, true))) return rc;
-  if (U2 && (rc = copy_field(h, ld, field_base(h, h->alloc_u2), es, U2, es, true))) return rc;
+  if (U1 && (rc = copy_field(h, ld, ubase(h, 0), es, U1, es, true))) return rc;
+  if (U2 && (rc = copy_field(h, ld, ubase(h, 1), es, U2, es, true))) return rc;
   if (U3 && P.roc2 && (rc = copy_field(h, ld, P.roc2, es, U3, es, true))) return rc;
   if (coef && P.coef) {
     const size_t hstride = (size_t)ld[0] * ld[1] * ld[2] * es;
@@ -351,8 +372,7 @@ extern "C" int gr_upload(gr_handle *h, const int ld[3], const void *U1, const vo
         return rc;
   }
   HIP_TRY(hipStreamSynchronize(h->stream));
-  h->parity = 0;
-  h->napplied = 0;
+  state_reset(h);
   return GR_OK;
 }
 
@@ -361,8 +381,8 @@ extern "C" int gr_download(gr_handle *h, const int ld[3], void *U1, void *U2) {
   int rc = check_ld(h, ld);
   if (rc) return rc;
   const size_t es = elem_size(h->P.dtype);
-  if (U1 && (rc = copy_field(h, ld, field_base(h, h->alloc_u1), es, U1, es, false))) return rc;
-  if (U2 && (rc = copy_field(h, ld, field_base(h, h->alloc_u2), es, U2, es, false))) return rc;
+  if (U1 && (rc = copy_field(h, ld, ubase(h, 0), es, U1, es, false))) return rc;
+  if (U2 && (rc = copy_field(h, ld, ubase(h, 1), es, U2, es, false))) return rc;
   HIP_TRY(hipStreamSynchronize(h->stream));
   return GR_OK;
 }
@@ -381,7 +401,7 @@ extern "C" int gr_set_source(gr_handle *h, int i, int j, int k, const double *va
   if (i < P.R || i >= P.nx + P.R || j < P.R || j >= P.ny + P.R || k < P.R || k >= P.nz + P.R)
     return fail(GR_EINVAL, "gr_set_source: point (%d,%d,%d) is not an interior cell", i, j, k);
   const size_t es = elem_size(P.dtype);
-  std::vector<char> buf((size_t)n * es);
+  std::vector<char> buf((size_t)(n + kSrcPad) * es, 0);  // zeros past n: fused passes read ahead
   for (int s = 0; s < n; ++s) {
     if (P.dtype == GR_F64)
       ((double *)buf.data())[s] = values[s];
@@ -395,22 +415,94 @@ extern "C" int gr_set_source(gr_handle *h, int i, int j, int k, const double *va
   return GR_OK;
 }
 
+static int check_halo_equal(gr_handle *h);
+static int ensure_spares(gr_handle *h);
+
+// One application U(w) <- f(U(r)) with the handle's sweep configuration.
+static hipError_t single_app(gr_handle *h, long long app) {
+  const size_t es = elem_size(h->P.dtype);
+  const bool has_src = h->src && app < h->nsrc;
+  const void *srcp = has_src ? (const char *)h->src + (size_t)app * es : nullptr;
+  const long long sidx = has_src ? h->src_idx : -1;
+  const int w = h->parity, r = 1 - w;
+  hipError_t e = sweep_fields(h->P, ubase(h, w), ubase(h, r), srcp, sidx, h->lc, h->stream);
+  h->parity ^= 1;
+  return e;
+}
+
+// Whether fused passes of depth tl may run for this handle.
+static int fused_depth_ok(gr_handle *h, int tl, bool *ok) {
+  *ok = false;
+  if (tl < 2) return GR_OK;
+  int depths[8];
+  const int nd = tb_depths(h->P.kernel, h->P.dtype, depths, 8);
+  bool have = false;
+  for (int i = 0; i < nd; ++i) have |= depths[i] == tl;
+  if (!have) return GR_OK;
+  if ((h->P.flags & GR_COEF_BF16) != 0) return GR_OK;
+  int rc = check_halo_equal(h);
+  if (rc) return rc;
+  *ok = h->halo_equal == 1;
+  return GR_OK;
+}
+
 extern "C" int gr_step(gr_handle *h, int napply) {
   if (!h) return fail(GR_EINVAL, "gr_step: null handle");
   if (napply < 0) return fail(GR_EINVAL, "gr_step: negative step count");
   const Problem &P = h->P;
-  void *b1 = field_base(h, h->alloc_u1), *b2 = field_base(h, h->alloc_u2);
   const size_t es = elem_size(P.dtype);
+  const int tl = h->lc.t_block;
+  bool fuse = false;
+  int rc = fused_depth_ok(h, tl, &fuse);
+  if (rc) return rc;
+  const bool wave = P.kernel == 0;
+  if (fuse && wave) {
+    if ((rc = ensure_spares(h))) return rc;
+    if (!h->spares_synced) {  // spares need the fields' halo (never written by a pass)
+      int sp[2];
+      spare_ids(h, sp);
+      const size_t fbytes = (size_t)P.field_elems * es;
+      HIP_TRY(hipMemcpyAsync(h->fbuf[sp[0]], h->fbuf[h->lu[1]], fbytes, hipMemcpyDeviceToDevice, h->stream));
+      HIP_TRY(hipMemcpyAsync(h->fbuf[sp[1]], h->fbuf[h->lu[1]], fbytes, hipMemcpyDeviceToDevice, h->stream));
+      h->spares_synced = true;
+    }
+  }
   HIP_TRY(hipEventRecord(h->ev0, h->stream));
-  for (int s = 0; s < napply; ++s) {
-    const long long app = h->napplied + s;
-    const bool has_src = h->src && app < h->nsrc;
-    const void *srcp = has_src ? (const char *)h->src + (size_t)app * es : nullptr;
-    const long long sidx = has_src ? h->src_idx : -1;
-    hipError_t e = (h->parity == 0) ? sweep_fields(P, b1, b2, srcp, sidx, h->lc, h->stream)
-                                    : sweep_fields(P, b2, b1, srcp, sidx, h->lc, h->stream);
+  int s = 0, launches = 0;
+  if (fuse) {
+    // kernel 0 keeps (u^(n-1), u^n) exact after every pass; first-order kernels
+    // keep the last application as a single sweep so U1 and U2 end exact.
+    const int reserve = wave ? 0 : 1;
+    while (napply - reserve - s >= tl) {
+      const long long app = h->napplied + s;
+      const bool has_src = h->src && app < h->nsrc;
+      const void *srcp = has_src ? (const char *)h->src + (size_t)app * es : nullptr;
+      const long long sidx = has_src ? h->src_idx : -1;
+      const int w = h->parity, r = 1 - w;
+      hipError_t e;
+      if (wave) {
+        int sp[2];
+        spare_ids(h, sp);
+        e = tblock_pass(P, tl, ubase(h, w), ubase(h, r), field_base(h, h->fbuf[sp[0]]), field_base(h, h->fbuf[sp[1]]),
+                        srcp, sidx, h->lc, h->stream);
+        h->parity ^= (tl & 1);
+        h->lu[h->parity] = sp[0];      // written next: holds level tl-1
+        h->lu[1 - h->parity] = sp[1];  // read next: level tl
+      } else {
+        const int pw = h->lu[w], pr = h->lu[r];
+        e = tblock_pass(P, tl, nullptr, ubase(h, r), nullptr, ubase(h, w), srcp, sidx, h->lc, h->stream);
+        h->parity ^= (tl & 1);
+        h->lu[1 - h->parity] = pw;  // newest (level tl) is read next
+        h->lu[h->parity] = pr;
+      }
+      if (e != hipSuccess) return fail(GR_EHIP, "fused pass launch failed: %s", hipGetErrorString(e));
+      s += tl;
+      ++launches;
+    }
+  }
+  for (; s < napply; ++s, ++launches) {
+    hipError_t e = single_app(h, h->napplied + s);
     if (e != hipSuccess) return fail(GR_EHIP, "sweep launch failed: %s", hipGetErrorString(e));
-    h->parity ^= 1;
   }
   HIP_TRY(hipEventRecord(h->ev1, h->stream));
   HIP_TRY(hipEventSynchronize(h->ev1));
@@ -419,7 +511,7 @@ extern "C" int gr_step(gr_handle *h, int napply) {
   h->napplied += napply;
   h->stats.last_ms = ms;
   h->stats.last_updates = (int64_t)napply * P.nx * P.ny * P.nz;
-  h->stats.last_launches = napply;
+  h->stats.last_launches = launches;
   return GR_OK;
 }
 
@@ -471,7 +563,14 @@ extern "C" int gr_set_config(gr_handle *h, int variant, int zchunk, int t_block)
   }
   if (zchunk >= 0) h->lc.zchunk = zchunk;
   if (t_block >= 1) {
-    if (t_block > 1) return fail(GR_EUNSUPPORTED, "gr_set_config: fused depth %d not available", t_block);
+    if (t_block > 1) {
+      int depths[8];
+      const int nd = tb_depths(h->P.kernel, h->P.dtype, depths, 8);
+      bool have = false;
+      for (int i = 0; i < nd; ++i) have |= depths[i] == t_block;
+      if (!have) return fail(GR_EUNSUPPORTED, "gr_set_config: fused depth %d not available for kernel %d", t_block,
+                             h->P.kernel);
+    }
     h->lc.t_block = t_block;
   }
   return GR_OK;
@@ -479,14 +578,13 @@ extern "C" int gr_set_config(gr_handle *h, int variant, int zchunk, int t_block)
 
 static int ensure_spares(gr_handle *h) {
   const size_t fbytes = (size_t)h->P.field_elems * elem_size(h->P.dtype);
-  if (!h->alloc_a) {
-    HIP_TRY(hipMalloc(&h->alloc_a, fbytes));
-    HIP_TRY(hipMemset(h->alloc_a, 0, fbytes));
-  }
-  if (!h->alloc_b) {
-    HIP_TRY(hipMalloc(&h->alloc_b, fbytes));
-    HIP_TRY(hipMemset(h->alloc_b, 0, fbytes));
-  }
+  for (int i = 2; i < 4; ++i)
+    if (!h->fbuf[i]) {
+      HIP_TRY(hipMalloc(&h->fbuf[i], fbytes));
+      HIP_TRY(hipMemset(h->fbuf[i], 0, fbytes));
+      h->spares_synced = false;
+      h->stats.device_bytes += (int64_t)fbytes;
+    }
   return GR_OK;
 }
 
@@ -497,9 +595,12 @@ extern "C" int gr_autotune(gr_handle *h, int steps, gr_stats *best) {
   if (rc) return rc;
   const Problem &P = h->P;
   const size_t fbytes = (size_t)P.field_elems * elem_size(P.dtype);
-  HIP_TRY(hipMemcpyAsync(h->alloc_a, h->alloc_u1, fbytes, hipMemcpyDeviceToDevice, h->stream));
-  HIP_TRY(hipMemcpyAsync(h->alloc_b, h->alloc_u2, fbytes, hipMemcpyDeviceToDevice, h->stream));
-  void *a = field_base(h, h->alloc_a), *b = field_base(h, h->alloc_b);
+  int sp[2];
+  spare_ids(h, sp);
+  HIP_TRY(hipMemcpyAsync(h->fbuf[sp[0]], h->fbuf[h->lu[0]], fbytes, hipMemcpyDeviceToDevice, h->stream));
+  HIP_TRY(hipMemcpyAsync(h->fbuf[sp[1]], h->fbuf[h->lu[1]], fbytes, hipMemcpyDeviceToDevice, h->stream));
+  h->spares_synced = true;  // copies of the fields: same halo
+  void *a = field_base(h, h->fbuf[sp[0]]), *b = field_base(h, h->fbuf[sp[1]]);
   const int nz = P.nz;
   std::vector<int> zchunks = {0};
   for (int zc : {512, 256, 128, 64, 32})
@@ -545,6 +646,40 @@ extern "C" int gr_autotune(gr_handle *h, int steps, gr_stats *best) {
       if ((rc = trial(lc))) return rc;
     }
   }
+  // stage 3: fused passes (time per application) on the scratch copies; kernel 0
+  // reads the fields themselves (never written here) and writes the scratch pair
+  {
+    int depths[8];
+    const int nd = tb_depths(P.kernel, P.dtype, depths, 8);
+    for (int i = 0; i < nd; ++i) {
+      bool ok = false;
+      if ((rc = fused_depth_ok(h, depths[i], &ok))) return rc;
+      if (!ok) continue;
+      const int tl = depths[i];
+      const int npass = (steps + tl - 1) / tl;
+      LaunchCfg lc = best_lc;
+      lc.t_block = tl;
+      auto pass = [&]() {
+        if (P.kernel == 0) return tblock_pass(P, tl, ubase(h, 0), ubase(h, 1), a, b, nullptr, -1, lc, h->stream);
+        return tblock_pass(P, tl, nullptr, b, nullptr, a, nullptr, -1, lc, h->stream);
+      };
+      hipError_t e = pass();
+      if (e != hipSuccess) return fail(GR_EHIP, "autotune fused launch: %s", hipGetErrorString(e));
+      HIP_TRY(hipEventRecord(h->ev0, h->stream));
+      for (int q = 0; q < npass; ++q)
+        if ((e = pass()) != hipSuccess) return fail(GR_EHIP, "autotune fused launch: %s", hipGetErrorString(e));
+      HIP_TRY(hipEventRecord(h->ev1, h->stream));
+      HIP_TRY(hipEventSynchronize(h->ev1));
+      float ms = 0.f;
+      HIP_TRY(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+      const float per_step = ms * steps / (float)(npass * tl);  // comparable with the sweep trials
+      if (per_step < best_ms) {
+        best_ms = per_step;
+        best_lc = lc;
+      }
+    }
+  }
+  h->spares_synced = true;  // the scratch pair holds tuning garbage in the interior; halos still match
   h->lc = best_lc;
   if (best) {
     rc = gr_get_stats(h, best);
@@ -557,12 +692,17 @@ extern "C" int gr_autotune(gr_handle *h, int steps, gr_stats *best) {
 extern "C" int gr_field_ptr(gr_handle *h, int which, void **p) {
   if (!h || !p) return fail(GR_EINVAL, "gr_field_ptr: null argument");
   switch (which) {
-    case 0: *p = h->alloc_u1; break;
-    case 1: *p = h->alloc_u2; break;
+    case 0: *p = h->fbuf[h->lu[0]]; break;
+    case 1: *p = h->fbuf[h->lu[1]]; break;
     case 2: *p = h->alloc_roc2; break;
     case 3: *p = h->alloc_coef; break;
-    case 4: *p = h->alloc_a; break;
-    case 5: *p = h->alloc_b; break;
+    case 4:
+    case 5: {
+      int sp[2];
+      spare_ids(h, sp);
+      *p = h->fbuf[sp[which - 4]];
+      break;
+    }
     default: return fail(GR_EINVAL, "gr_field_ptr: bad field %d", which);
   }
   return GR_OK;
@@ -672,6 +812,22 @@ __global__ void init_fields_kernel(T *u1, T *u2, T *r2, int mode, uint64_t seed,
   u2[o] = val;
 }
 
+// flag = 1 if U1 and U2 differ (bitwise) anywhere outside the interior box
+template <typename U>
+__global__ void halo_diff_kernel(const U *a, const U *b, int nx, int ny, int nz, int R, long long pitch, long long rows,
+                                 long long total, int *flag) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= total) return;
+  const int nnx = nx + 2 * R;
+  const long long row = t / nnx;
+  const int i = (int)(t - row * nnx);
+  const int j = (int)(row % rows);
+  const int k = (int)(row / rows);
+  const bool inner = i >= R && i < nx + R && j >= R && j < ny + R && k >= R && k < nz + R;
+  const long long o = row * pitch + i;
+  if (!inner && a[o] != b[o]) *flag = 1;
+}
+
 template <typename C>
 __global__ void init_coef_kernel(C *coef, int ncoef, uint64_t seed, double lo0, double hi0, double lo1, double hi1,
                                  long long field_elems) {
@@ -686,6 +842,30 @@ __global__ void init_coef_kernel(C *coef, int ncoef, uint64_t seed, double lo0, 
 
 }  // namespace
 
+static int check_halo_equal(gr_handle *h) {
+  if (h->halo_equal >= 0) return GR_OK;
+  const Problem &P = h->P;
+  if (!h->flag) HIP_TRY(hipMalloc(&h->flag, sizeof(int)));
+  HIP_TRY(hipMemsetAsync(h->flag, 0, sizeof(int), h->stream));
+  const long long total = (long long)(P.nx + 2 * P.R) * P.rows * P.planes;
+  const int bs = 256;
+  const unsigned nb = (unsigned)((total + bs - 1) / bs);
+  if (P.dtype == GR_F64)
+    hipLaunchKernelGGL(halo_diff_kernel<unsigned long long>, dim3(nb), dim3(bs), 0, h->stream,
+                       (const unsigned long long *)ubase(h, 0), (const unsigned long long *)ubase(h, 1), P.nx, P.ny,
+                       P.nz, P.R, (long long)P.pitch, (long long)P.rows, total, (int *)h->flag);
+  else
+    hipLaunchKernelGGL(halo_diff_kernel<unsigned>, dim3(nb), dim3(bs), 0, h->stream, (const unsigned *)ubase(h, 0),
+                       (const unsigned *)ubase(h, 1), P.nx, P.ny, P.nz, P.R, (long long)P.pitch, (long long)P.rows,
+                       total, (int *)h->flag);
+  HIP_TRY(hipGetLastError());
+  int hflag = 1;
+  HIP_TRY(hipMemcpyAsync(&hflag, h->flag, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  h->halo_equal = hflag == 0 ? 1 : 0;
+  return GR_OK;
+}
+
 extern "C" int gr_init_fields(gr_handle *h, int mode, uint64_t seed) {
   if (!h) return fail(GR_EINVAL, "gr_init_fields: null handle");
   if (mode < 0 || mode > 2) return fail(GR_EINVAL, "gr_init_fields: bad mode %d", mode);
@@ -693,7 +873,7 @@ extern "C" int gr_init_fields(gr_handle *h, int mode, uint64_t seed) {
   const long long total = (long long)(P.nx + 2 * P.R) * P.rows * P.planes;
   const int bs = 256;
   const unsigned nb = (unsigned)((total + bs - 1) / bs);
-  void *b1 = field_base(h, h->alloc_u1), *b2 = field_base(h, h->alloc_u2);
+  void *b1 = ubase(h, 0), *b2 = ubase(h, 1);
   if (P.dtype == GR_F64) {
     hipLaunchKernelGGL(init_fields_kernel<double>, dim3(nb), dim3(bs), 0, h->stream, (double *)b1, (double *)b2,
                        (P.roc2 && mode == 0) ? (double *)P.roc2 : nullptr, mode, seed, P.nx, P.ny, P.nz, P.R,
@@ -707,8 +887,7 @@ extern "C" int gr_init_fields(gr_handle *h, int mode, uint64_t seed) {
   }
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipStreamSynchronize(h->stream));
-  h->parity = 0;
-  h->napplied = 0;
+  state_reset(h);
   return GR_OK;
 }
 
@@ -790,8 +969,8 @@ extern "C" int gr_compare_fields(gr_handle *a, int fa, gr_handle *b, int fb, dou
   const Problem &A = a->P, &B = b->P;
   if (A.nx != B.nx || A.ny != B.ny || A.nz != B.nz || A.R != B.R)
     return fail(GR_EINVAL, "gr_compare_fields: grids differ");
-  const void *pa = field_base(a, fa == 0 ? a->alloc_u1 : a->alloc_u2);
-  const void *pb = field_base(b, fb == 0 ? b->alloc_u1 : b->alloc_u2);
+  const void *pa = ubase(a, fa);
+  const void *pb = ubase(b, fb);
   const int nblk = 2048;
   double *part = nullptr;
   HIP_TRY(hipMalloc(&part, sizeof(double) * 4 * nblk));

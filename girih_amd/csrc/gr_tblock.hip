@@ -1,0 +1,377 @@
+// gr_tblock.hip -- temporally blocked sweeps: TL time steps per HBM pass.
+//
+// The GPU analogue of girih's diamond + wavefront temporal blocking
+// (src/kernels/diamond_ts.c, stencils_1wf.ic:19-82): instead of diamonds in y-t
+// and a wavefront in z run by a CPU thread, a workgroup owns an xy tile and
+// streams z once, computing TL time levels in a z pipeline.  Level t trails level
+// t-1 by R planes -- the `kt -= NHALO` of stencils_1wf.ic:77 -- so every level
+// finds the planes it needs already computed on chip:
+//
+//   step k0:  level 0 = plane k0 of the input enters the pipeline,
+//             level t computes plane k0 - R*t   (t = 1 .. TL)
+//
+// Level t is computed over the tile plus an xy margin of R*(TL - t) cells (the
+// overlapped ghost zone that makes the tile independent of its neighbours).  A
+// lane owns columns of the level-1 region; for each level it keeps a register
+// queue of the 2R+1 most recent planes (the z neighbours of the next level and,
+// for the second-order wave, the u^(n-1) term two levels up), and the current
+// plane of each level is staged in LDS for the xy neighbours of the next level.
+// Only the input field(s) are read from HBM and only the tile's output planes are
+// written, so a pass moves (in + out) words per cell for TL updates.
+//
+// Boundary cells (girih's halo, never updated) keep their input value at every
+// level; the host enables this path only when U1 and U2 hold the same halo.
+//
+// First-order kernels (1, 5): level t = f(level t-1); input Y, output Z.
+// Kernel 0 (u^(n+1) = 2u^n - u^(n-1) + roc2 Lap u^n): inputs X = u^(n-1),
+// Y = u^n; outputs ZO = level TL-1, ZN = level TL.
+#include <hip/hip_runtime.h>
+
+#include "../../include/girih_gpu.h"
+#include "gr_internal.hpp"
+#include "gr_sweep.hpp"
+
+namespace gr {
+
+template <typename T>
+struct TbArgs {
+  const T *X;   // kernel 0: u^(n-1)
+  const T *Y;   // level 0
+  T *ZO;        // kernel 0: level TL-1
+  T *ZN;        // level TL
+  const T *roc2;
+  const T *coef;
+  long long cstride;
+  T c[5];
+  T csum, roc2s;
+  long long sy, sz;
+  int nx, ny, nz;  // interior; boxes are [R, n+R)
+  int tiles_x, tiles_y, xcd_order;
+  const T *src;    // source values of this pass (level t adds src[t-1]); src_k < 0: none
+  int src_k, src_off;
+};
+
+template <typename T, int KID, int NT, int TXO, int TYO, int TL, bool FAST, bool ROCS>
+__global__ __launch_bounds__(NT) void tb_kernel(const TbArgs<T> a) {
+  constexpr int R = kradius(KID);
+  constexpr int Q = 2 * R + 1;
+  constexpr int NC = kncoef(KID);
+  constexpr int W0 = TXO + 2 * R * TL, H0 = TYO + 2 * R * TL;          // level-0 region
+  constexpr int W1 = TXO + 2 * R * (TL - 1), H1 = TYO + 2 * R * (TL - 1);  // level-1 region
+  constexpr int NCOL = (W1 * H1 + NT - 1) / NT;                            // columns per lane
+  constexpr int NH = W0 * H0 - W1 * H1;                                    // level-0 halo ring
+  constexpr int HS = (NH + NT - 1) / NT;
+  constexpr int PL = W0 * H0;  // one LDS plane (region-0 coordinates for every level)
+  __shared__ T lds[TL][PL];
+
+  const int nb = a.tiles_x * a.tiles_y;
+  const int lb = a.xcd_order ? xcd_remap(blockIdx.x, nb) : (int)blockIdx.x;
+  const int tx0 = R + (lb % a.tiles_x) * TXO;  // tile origin (girih interior indices)
+  const int ty0 = R + (lb / a.tiles_x) * TYO;
+  const int nnx = a.nx + 2 * R, nny = a.ny + 2 * R, nnz = a.nz + 2 * R;
+  const int tid = threadIdx.x;
+
+  // lane columns in region-1 coordinates
+  unsigned off[NCOL];   // in-plane byte offset (clamped into the field)
+  int lofs[NCOL];       // LDS index (region-0 coordinates)
+  int cxv[NCOL], cyv[NCOL];
+  bool inner[NCOL], outc[NCOL], colv[NCOL];
+#pragma unroll
+  for (int j = 0; j < NCOL; ++j) {
+    const int c = min(tid + j * NT, W1 * H1 - 1);
+    const int cx = c % W1, cy = c / W1;
+    colv[j] = tid + j * NT < W1 * H1;  // the last lane slots may be spare
+    cxv[j] = colv[j] ? cx : -1000000;
+    cyv[j] = cy;
+    const int gx = tx0 - R * (TL - 1) + cx, gy = ty0 - R * (TL - 1) + cy;
+    const int gxc = min(max(gx, 0), nnx - 1), gyc = min(max(gy, 0), nny - 1);
+    off[j] = (unsigned)(gyc * (int)a.sy + gxc) * (unsigned)sizeof(T);
+    lofs[j] = (cy + R) * W0 + (cx + R);
+    inner[j] = gx >= R && gx < a.nx + R && gy >= R && gy < a.ny + R;
+    // output: tile cells inside the interior
+    outc[j] = (tid + j * NT < W1 * H1) && cx >= R * (TL - 1) && cx < R * (TL - 1) + TXO &&
+              cy >= R * (TL - 1) && cy < R * (TL - 1) + TYO && inner[j];
+  }
+  // level-0 halo ring slots
+  unsigned hoff[HS];
+  int hl[HS];
+  bool hw[HS];
+#pragma unroll
+  for (int s = 0; s < HS; ++s) {
+    const int e = min(tid + s * NT, NH - 1);
+    hw[s] = tid + s * NT < NH;
+    // ring: R*TL... enumerate region 0 cells that are not in region 1
+    int lx, ly;
+    {
+      const int top = R * W0;  // R rows above region 1
+      if (e < top) {
+        ly = e / W0;
+        lx = e % W0;
+      } else if (e < 2 * top) {
+        ly = R + H1 + (e - top) / W0;
+        lx = (e - top) % W0;
+      } else {
+        const int d = e - 2 * top, side = d / (R * H1), f = d - side * R * H1;
+        ly = R + f / R;
+        lx = (f % R) + side * (R + W1);
+      }
+    }
+    const int gx = tx0 - R * TL + lx, gy = ty0 - R * TL + ly;
+    const int gxc = min(max(gx, 0), nnx - 1), gyc = min(max(gy, 0), nny - 1);
+    hoff[s] = (unsigned)(gyc * (int)a.sy + gxc) * (unsigned)sizeof(T);
+    hl[s] = ly * W0 + lx;
+  }
+
+  auto planeY = [&](int k) { return a.Y + (long long)k * a.sz; };
+  T c[5];
+#pragma unroll
+  for (int r = 0; r < 5; ++r) c[r] = a.c[r];
+
+  // queues: qv[t][j][slot], level t plane p in slot (p mod Q)
+  T qv[TL][NCOL][Q];
+#pragma unroll
+  for (int t = 0; t < TL; ++t)
+#pragma unroll
+    for (int j = 0; j < NCOL; ++j)
+#pragma unroll
+      for (int i = 0; i < Q; ++i) qv[t][j][i] = T(0);
+
+  const int kfirst = -R * TL, klast = a.nz + 2 * R + R * TL;  // k0 range (whole z extent)
+  // prefetch level-0 plane kfirst (only planes inside [0, nnz) exist)
+  T n0[NCOL], nh[HS];
+  // interior columns of plane k (queue entry) and the halo ring of plane k - R
+  // (the level-0 plane that level 1 centres on when plane k enters the pipeline)
+  auto fetch0 = [&](int k) {
+    const bool ok = k >= 0 && k < nnz, okr = k - R >= 0 && k - R < nnz;
+    const T *vp = planeY(min(max(k, 0), nnz - 1));
+    const T *rp = planeY(min(max(k - R, 0), nnz - 1));
+#pragma unroll
+    for (int j = 0; j < NCOL; ++j) n0[j] = ok ? ldb<T>(vp, off[j]) : T(0);
+#pragma unroll
+    for (int s = 0; s < HS; ++s) nh[s] = okr ? ldb<T>(rp, hoff[s]) : T(0);
+  };
+  fetch0(kfirst);
+
+  for (int kb = kfirst; kb < klast; kb += Q) {
+#pragma unroll
+    for (int ph = 0; ph < Q; ++ph) {
+      const int k0 = kb + ph;
+      if (k0 >= klast) break;
+      // slot of plane p: ((p mod Q) + Q) mod Q; k0 - kfirst = kb - kfirst + ph and
+      // (kb - kfirst) is a multiple of Q, so plane k0 - R*t sits in slot
+      // (ph - R*t) mod Q relative to kfirst's slot 0.
+      // ---- level 0: plane k0 enters the queue; LDS gets plane k0 - R, the centre
+      // plane of level 1 in this step (from the queue, R steps old) and its halo ring
+#pragma unroll
+      for (int j = 0; j < NCOL; ++j) qv[0][j][ph] = n0[j];
+      {
+        T *L0 = lds[0];
+#pragma unroll
+        for (int j = 0; j < NCOL; ++j)
+          if (colv[j]) L0[lofs[j]] = qv[0][j][((ph - R) % Q + Q) % Q];
+#pragma unroll
+        for (int s = 0; s < HS; ++s)
+          if (hw[s]) L0[hl[s]] = nh[s];
+      }
+      fetch0(k0 + 1);
+      __syncthreads();
+
+#pragma unroll
+      for (int t = 1; t <= TL; ++t) {
+        const int p = k0 - R * t;  // plane computed by level t
+        const bool live = p >= 0 && p < nnz;
+        const bool zin = p >= R && p < a.nz + R;
+        const int sc = ((ph - R * t) % Q + Q) % Q;  // slot of plane p in level t-1's queue
+        T val[NCOL];
+        if (live) {
+          const T *Lp = lds[t - 1];
+          // kernel 0: u^(n+t-2) at plane p -- level t-2's queue, or X for t = 1
+          T uo[NCOL];
+          if constexpr (KID == 0) {
+            if (t == 1) {
+              const T *xp = a.X + (long long)p * a.sz;
+#pragma unroll
+              for (int j = 0; j < NCOL; ++j) uo[j] = ldb<T>(xp, off[j]);
+            } else {
+              const int so = ((ph - R * t) % Q + Q) % Q;  // plane p in level t-2's queue
+#pragma unroll
+              for (int j = 0; j < NCOL; ++j) uo[j] = qv[t - 2 >= 0 ? t - 2 : 0][j][so];
+            }
+          }
+          const T *yp = planeY(p);
+#pragma unroll
+          for (int j = 0; j < NCOL; ++j) {
+            // region of level t in region-1 coordinates: [R(t-1), W1-R(t-1))
+            const bool inreg = cxv[j] >= R * (t - 1) && cxv[j] < W1 - R * (t - 1) && cyv[j] >= R * (t - 1) &&
+                               cyv[j] < H1 - R * (t - 1);
+            if (!inreg) {
+              val[j] = T(0);
+              continue;
+            }
+            T out;
+            if (inner[j] && zin) {
+              const T *q = qv[t - 1][j];
+              auto zn = [&](int d) { return q[((sc + d) % Q + Q) % Q]; };
+              auto xn = [&](int d) { return Lp[lofs[j] + d]; };
+              auto yn = [&](int d) { return Lp[lofs[j] + d * W0]; };
+              T zq[Q];
+#pragma unroll
+              for (int d = -R; d <= R; ++d) zq[R + d] = zn(d);
+              const T v0 = zq[R];
+              if constexpr (KID == 0) {
+                const T roc = ROCS ? a.roc2s : a.roc2[(long long)p * a.sz + off[j] / sizeof(T)];
+                out = Op<T, 0, FAST>::update(v0, uo[j], roc,
+                                             Op<T, 0, FAST>::template lap<2>(c, a.csum, v0, zq, xn, yn));
+              } else if constexpr (KID == 1) {  // stencils.c:73-78
+                out = c[0] * v0 + c[1] * (xn(1) + xn(-1)) + c[1] * (yn(1) + yn(-1)) + c[1] * (zq[2] + zq[0]);
+              } else {  // KID == 5, stencils.c:193-201
+                const T *cp = a.coef + (long long)p * a.sz + off[j] / sizeof(T);
+                out = cp[0] * v0 + cp[a.cstride] * xn(-1) + cp[2 * a.cstride] * xn(1) +
+                      cp[3 * a.cstride] * yn(-1) + cp[4 * a.cstride] * yn(1) + cp[5 * a.cstride] * zq[0] +
+                      cp[6 * a.cstride] * zq[2];
+              }
+              if (p == a.src_k && off[j] == (unsigned)a.src_off * (unsigned)sizeof(T)) out = out + a.src[t - 1];
+            } else {
+              out = ldb<T>(yp, off[j]);  // girih's halo: never updated
+            }
+            val[j] = out;
+          }
+        } else {
+#pragma unroll
+          for (int j = 0; j < NCOL; ++j) val[j] = T(0);
+        }
+        // level t: plane p into its queue; the LDS gets its plane p - R (the centre
+        // plane of level t+1 in this step), taken from the queue
+        if (t < TL) {
+          const int sw = ((ph - R * t) % Q + Q) % Q;
+          const int sl = ((ph - R * t - R) % Q + Q) % Q;
+          T *Lt = lds[t < TL ? t : 0];
+#pragma unroll
+          for (int j = 0; j < NCOL; ++j) qv[t < TL ? t : 0][j][sw] = val[j];
+#pragma unroll
+          for (int j = 0; j < NCOL; ++j)
+            if (colv[j]) Lt[lofs[j]] = qv[t < TL ? t : 0][j][sl];
+        }
+        // outputs: level TL into ZN, level TL-1 into ZO (kernel 0) -- planes of the box
+        if (live && zin) {
+          if (t == TL) {
+#pragma unroll
+            for (int j = 0; j < NCOL; ++j)
+              if (outc[j]) stb<T>(a.ZN + (long long)p * a.sz, off[j], val[j]);
+          } else if (KID == 0 && t == TL - 1) {
+#pragma unroll
+            for (int j = 0; j < NCOL; ++j)
+              if (outc[j]) stb<T>(a.ZO + (long long)p * a.sz, off[j], val[j]);
+          }
+        }
+        if (t < TL) __syncthreads();
+      }
+    }
+  }
+}
+
+template <typename T, int KID, int NT, int TXO, int TYO, int TL, bool FAST, bool ROCS>
+static hipError_t launch_tb(const TbArgs<T> &a0, hipStream_t st) {
+  TbArgs<T> a = a0;
+  a.tiles_x = (a.nx + TXO - 1) / TXO;
+  a.tiles_y = (a.ny + TYO - 1) / TYO;
+  hipLaunchKernelGGL((tb_kernel<T, KID, NT, TXO, TYO, TL, FAST, ROCS>), dim3(a.tiles_x * a.tiles_y), dim3(NT), 0, st,
+                     a);
+  return hipGetLastError();
+}
+
+// Fused depths available per kernel (0 = not available)
+int tb_depths(int kid, int dtype, int *out, int max) {
+  int n = 0;
+  auto add = [&](int d) {
+    if (n < max) out[n++] = d;
+  };
+  (void)dtype;
+  if (kid == 1 || kid == 5) {
+    add(2);
+    add(4);
+    add(6);
+  } else if (kid == 0) {
+    add(2);
+  }
+  return n;
+}
+
+// Tiles per kernel and precision, sized so the per-lane queues (levels x columns
+// x (2R+1) words) stay in registers without spilling.
+template <typename T>
+static hipError_t tb_dispatch(int kid, int tl, bool fast, bool rocs, const TbArgs<T> &a, hipStream_t st) {
+  constexpr bool F32 = std::is_same<T, float>::value;
+  constexpr int TYS = F32 ? 32 : 16;  // first-order kernels: 64 x TYS output tile, 512 lanes
+  if (kid == 1) {
+    if (tl == 2) return launch_tb<T, 1, 512, 64, TYS, 2, false, false>(a, st);
+    if (tl == 4) return launch_tb<T, 1, 512, 64, TYS, 4, false, false>(a, st);
+    if (tl == 6) return launch_tb<T, 1, 512, 64, 16, 6, false, false>(a, st);
+  } else if (kid == 5) {
+    if (tl == 2) return launch_tb<T, 5, 512, 64, TYS, 2, false, false>(a, st);
+    if (tl == 4) return launch_tb<T, 5, 512, 64, TYS, 4, false, false>(a, st);
+    if (tl == 6) return launch_tb<T, 5, 512, 64, 16, 6, false, false>(a, st);
+  } else if (kid == 0 && tl == 2) {  // 64 x 16 output tile, 1024 lanes
+    if constexpr (F32) {
+      if (fast) {
+        if (rocs) return launch_tb<T, 0, 1024, 64, 16, 2, true, true>(a, st);
+        return launch_tb<T, 0, 1024, 64, 16, 2, true, false>(a, st);
+      }
+    }
+    if (rocs) return launch_tb<T, 0, 1024, 64, 16, 2, false, true>(a, st);
+    return launch_tb<T, 0, 1024, 64, 16, 2, false, false>(a, st);
+  }
+  return hipErrorInvalidValue;
+}
+
+// One fused pass of tl levels (see header comment for the field roles).
+hipError_t tblock_pass(const Problem &P, int tl, const void *X, const void *Y, void *ZO, void *ZN, const void *src,
+                       long long src_idx, const LaunchCfg &lc, hipStream_t st) {
+  const bool fast = (P.flags & GR_FAST) != 0, rocs = (P.flags & GR_ROC2_SCALAR) != 0;
+  const long long sz = (long long)P.pitch * P.rows;
+  if (P.dtype == GR_F64) {
+    TbArgs<double> a{};
+    a.X = (const double *)X;
+    a.Y = (const double *)Y;
+    a.ZO = (double *)ZO;
+    a.ZN = (double *)ZN;
+    a.roc2 = (const double *)P.roc2;
+    a.coef = (const double *)P.coef;
+    a.cstride = P.field_elems;
+    for (int r = 0; r < 5; ++r) a.c[r] = P.c[r];
+    a.roc2s = P.roc2s;
+    a.sy = P.pitch;
+    a.sz = sz;
+    a.nx = P.nx;
+    a.ny = P.ny;
+    a.nz = P.nz;
+    a.xcd_order = lc.xcd_order;
+    a.src = (const double *)src;
+    a.src_k = src_idx < 0 ? -1 : (int)(src_idx / sz);
+    a.src_off = src_idx < 0 ? 0 : (int)(src_idx % sz);
+    return tb_dispatch<double>(P.kernel, tl, false, rocs, a, st);
+  }
+  TbArgs<float> a{};
+  a.X = (const float *)X;
+  a.Y = (const float *)Y;
+  a.ZO = (float *)ZO;
+  a.ZN = (float *)ZN;
+  a.roc2 = (const float *)P.roc2;
+  a.coef = (const float *)P.coef;
+  a.cstride = P.field_elems;
+  for (int r = 0; r < 5; ++r) a.c[r] = (float)P.c[r];
+  a.csum = (float)(P.c[0] + 6.0 * (P.c[1] + P.c[2]));
+  a.roc2s = (float)P.roc2s;
+  a.sy = P.pitch;
+  a.sz = sz;
+  a.nx = P.nx;
+  a.ny = P.ny;
+  a.nz = P.nz;
+  a.xcd_order = lc.xcd_order;
+  a.src = (const float *)src;
+  a.src_k = src_idx < 0 ? -1 : (int)(src_idx / sz);
+  a.src_off = src_idx < 0 ? 0 : (int)(src_idx % sz);
+  return tb_dispatch<float>(P.kernel, tl, fast, rocs, a, st);
+}
+
+}  // namespace gr

@@ -1,0 +1,142 @@
+"""Temporally blocked (fused) passes: several time steps per HBM pass.
+
+The fused kernels keep girih's operand order, so FP64/FP32 results must stay
+bit-identical to the CPU oracle (and to single sweeps) for every depth, ragged
+size, step count and source -- including the final U1/U2 pair.
+"""
+import numpy as np
+import pytest
+
+import oracle as orc
+from girih_amd import gpu
+
+from test_gpu_parity import _consts, _pad, _pad_coef, _random_problem
+
+pytestmark = pytest.mark.gpu
+
+SIZES = {1: [(70, 38, 20), (130, 9, 11), (64, 32, 8)], 4: [(72, 20, 17), (67, 33, 12)]}
+
+
+@pytest.mark.parametrize("kernel,depth", [(1, 2), (1, 4), (1, 6), (5, 2), (5, 4), (5, 6), (0, 2)])
+@pytest.mark.parametrize("prec", ["dp", "sp"])
+def test_fused_bitexact_vs_oracle(kernel, depth, prec):
+    dt = np.float64 if prec == "dp" else np.float32
+    dtype = gpu.GR_F64 if prec == "dp" else gpu.GR_F32
+    for n, (nx, ny, nz) in enumerate(SIZES[orc.RADIUS[kernel]]):
+        for napply in (2 * depth + 1, 3 * depth):
+            print("fused", kernel, depth, prec, (nx, ny, nz), napply, flush=True)
+            u, v, roc2, coef = _random_problem(kernel, nx, ny, nz, dt, 1000 + 10 * kernel + n)
+            with gpu.Stepper(kernel, nx, ny, nz, dtype=dtype, coef=_consts(coef, kernel)) as st:
+                st.upload(_pad(u), _pad(v), _pad(roc2) if roc2 is not None else None, _pad_coef(coef, kernel))
+                st.set_config(t_block=depth)
+                st.step(napply)
+                s = st.stats()
+                assert s["last_launches"] < napply, "fused passes were not used"
+                U1, U2 = st.download(_pad(u).shape)
+            for a in range(napply):
+                if a % 2 == 0:
+                    orc.sweep(kernel, coef, u, v, roc2)
+                else:
+                    orc.sweep(kernel, coef, v, u, roc2)
+            nnx = u.shape[2]
+            np.testing.assert_array_equal(U1[:, :, :nnx], u)
+            np.testing.assert_array_equal(U2[:, :, :nnx], v)
+
+
+def test_fused_wave_with_source_fp64():
+    nx, ny, nz = 66, 40, 30
+    kernel = 0
+    shp = orc.shape_of(kernel, nx, ny, nz)
+    u = np.zeros(shp)
+    v = np.zeros(shp)
+    roc2 = np.full(shp, 0.16)
+    coef = np.zeros(11)
+    w = [-205.0 / 72, 8.0 / 5, -1.0 / 5, 8.0 / 315, -1.0 / 560]
+    coef[0] = 3 * w[0]
+    coef[1:5] = w[1:5]
+    src = np.cos(np.arange(16) * 0.9) + 0.2
+    pt = (4 + 40, 4 + 20, 4 + 15)
+    with gpu.Stepper(kernel, nx, ny, nz, dtype=gpu.GR_F64, coef=list(coef[:5])) as st:
+        st.upload(_pad(u), _pad(v), _pad(roc2))
+        st.set_source(*pt, src[:13])
+        st.set_config(t_block=2)
+        st.step(16)  # source values past 13 add nothing
+        U1, U2 = st.download(_pad(u).shape)
+    s2 = np.zeros(16)
+    s2[:13] = src[:13]
+    orc.run(kernel, coef, u, v, roc2, nt=16, src_idx=pt, src=s2)
+    np.testing.assert_array_equal(U1[:, :, :u.shape[2]], u)
+    np.testing.assert_array_equal(U2[:, :, :u.shape[2]], v)
+
+
+def test_fused_fast_fp32_equals_single_sweep_fast():
+    """GR_FAST fused passes use the same arithmetic as GR_FAST single sweeps."""
+    nx, ny, nz = 80, 36, 24
+    kernel = 0
+    u, v, _, coef = _random_problem(kernel, nx, ny, nz, np.float32, 77)
+    outs = []
+    for depth in (1, 2):
+        with gpu.Stepper(kernel, nx, ny, nz, dtype=gpu.GR_F32, flags=gpu.GR_FAST | gpu.GR_ROC2_SCALAR, roc2=0.12,
+                         coef=list(coef[:5])) as st:
+            st.upload(_pad(u), _pad(v))
+            st.set_config(t_block=depth)
+            st.step(9)
+            outs.append(st.download(_pad(u).shape))
+    np.testing.assert_array_equal(outs[0][0], outs[1][0])
+    np.testing.assert_array_equal(outs[0][1], outs[1][1])
+
+
+def test_unequal_halos_fall_back_to_single_sweeps():
+    """Fused passes assume U1 and U2 share girih's halo; otherwise the stepper must
+    quietly use single sweeps and stay exact."""
+    kernel, nx, ny, nz = 1, 40, 20, 12
+    u, v, _, coef = _random_problem(kernel, nx, ny, nz, np.float64, 3)
+    v[0, :, :] = 1.5  # a halo plane of U2 differs from U1's
+    with gpu.Stepper(kernel, nx, ny, nz, dtype=gpu.GR_F64, coef=_consts(coef, kernel)) as st:
+        st.upload(_pad(u), _pad(v))
+        st.set_config(t_block=4)
+        st.step(10)
+        assert st.stats()["last_launches"] == 10
+        U1, U2 = st.download(_pad(u).shape)
+    orc.run(kernel, coef, u, v, None, nt=10)
+    np.testing.assert_array_equal(U1[:, :, :u.shape[2]], u)
+    np.testing.assert_array_equal(U2[:, :, :u.shape[2]], v)
+
+
+def test_girih_init_fused_matches_golden_semantics():
+    """girih's closed-form state, diamond-stepper semantics (nt-1 applications),
+    run with fused passes: U1/U2 equal the oracle's."""
+    kernel, nx, ny, nz, nt, t_dim = 1, 16, 32, 24, 9, 3
+    u, v, _ = orc.init_ref(kernel, nx, ny, nz)
+    coef = orc.init_coef(kernel, u.shape)
+    with gpu.Stepper(kernel, nx, ny, nz, dtype=gpu.GR_F64, coef=_consts(coef, kernel)) as st:
+        st.init_fields(0)
+        st.set_config(t_block=4)
+        ntr = st.run_ts(2, nt, t_dim)
+        U1, U2 = st.download(_pad(u).shape)
+    orc.run(kernel, coef, u, v, None, nt=ntr - 2)
+    orc.sweep(kernel, coef, u, v)
+    np.testing.assert_array_equal(U1[:, :, :u.shape[2]], u)
+    np.testing.assert_array_equal(U2[:, :, :u.shape[2]], v)
+
+
+def test_autotune_keeps_fields_and_may_pick_fused():
+    kernel, nx, ny, nz = 5, 96, 64, 40
+    u, v, _, coef = _random_problem(kernel, nx, ny, nz, np.float32, 9)
+    with gpu.Stepper(kernel, nx, ny, nz, dtype=gpu.GR_F32) as st:
+        st.upload(_pad(u), _pad(v), None, _pad_coef(coef, kernel))
+        best = st.autotune(4)
+        assert best["t_block"] in (1, 2, 4, 6)
+        st.step(7)
+        U1, U2 = st.download(_pad(u).shape)
+    orc.run(kernel, coef, u, v, None, nt=6)
+    orc.sweep(kernel, coef, u, v)
+    np.testing.assert_array_equal(U1[:, :, :u.shape[2]], u)
+    np.testing.assert_array_equal(U2[:, :, :u.shape[2]], v)
+
+
+def test_set_config_rejects_unavailable_depth():
+    with gpu.Stepper(1, 16, 16, 16) as st:
+        with pytest.raises(gpu.GirihError) as ei:
+            st.set_config(t_block=3)
+        assert ei.value.code == gpu.GR_EUNSUPPORTED
