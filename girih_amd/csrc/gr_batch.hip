@@ -1,0 +1,449 @@
+// gr_batch.hip -- many independent kernel-0 wave problems of uneven size, one
+// persistent launch per time step, the step loop optionally replayed from a HIP
+// graph.
+//
+// girih steps one grid at a time and balances its CPU thread groups with a
+// dynamic ready queue of diamond tiles (src/kernels/diamond_ts.c:663-747,
+// `avail_list` under omp critical).  Here the queue spans grids: the work items
+// of one time step are the (grid, xy tile, z chunk) triples of every problem,
+// ordered largest grid first; a grid of persistent workgroups (as many as the
+// GPU holds at once) claims them with one atomic increment each, so large and
+// small grids share the CUs without a launch per grid and without a tail of idle
+// CUs behind the largest grid.  Each item streams its tile through its z chunk
+// with the single-sweep body of gr_sweep.hpp (SweepTile::run).
+//
+// The claim counter is never reset between the steps of one call: step s claims
+// values in [s*(items+grid), (s+1)*(items+grid)), because every workgroup makes
+// exactly one failing claim per step.  One reset kernel (not a memset, so the
+// sequence can live in a HIP graph) starts each call.
+//
+// Fields are FP32 with GR_FAST arithmetic; roc2 is a per-cell field stored as
+// bf16 (GR_COEF_BF16) or FP32.
+#include <hip/hip_runtime.h>
+#include <hip/hip_bf16.h>
+
+#include <algorithm>
+#include <numeric>
+#include <vector>
+
+#include "../../include/girih_gpu.h"
+#include "gr_internal.hpp"
+#include "gr_sweep.hpp"
+
+namespace gr {
+namespace {
+
+// One problem of one step parity.  All words are 4-byte multiples (read with
+// readfirstlane into scalar registers).
+struct BatchDesc {
+  float *u;          // written: u^(n+1) (holds u^(n-1) on entry)
+  const float *v;    // read: u^n
+  const void *roc2;  // bf16 or FP32 roc2 field (x offset applied)
+  const float *src;  // source values of this call's first step (null = none)
+  long long sy, sz;
+  int xe, ye, ze;    // interior box ends (the box starts at R = 4)
+  int tiles_x, tiles_y, nzc, zchunk;
+  int item0;         // first work item of this problem
+  int nsrc;          // source values available from `src`
+  int src_k, src_off;
+  float c[5], csum;
+  int pad;
+};
+static_assert(sizeof(BatchDesc) % 8 == 0, "descriptor words");
+
+constexpr int kMaxBatch = 256;
+
+__global__ void batch_reset_kernel(unsigned long long *ctr) {
+  if (threadIdx.x == 0 && threadIdx.y == 0) *ctr = 0ull;
+}
+
+template <typename X>
+__device__ __forceinline__ X uniform_load(const X *p) {
+  constexpr int W = sizeof(X) / 4;
+  int w[W];
+  const int *s = reinterpret_cast<const int *>(p);
+#pragma unroll
+  for (int i = 0; i < W; ++i) w[i] = __builtin_amdgcn_readfirstlane(s[i]);
+  X x;
+  __builtin_memcpy(&x, w, sizeof(X));
+  return x;
+}
+
+template <typename C, int BY, int RY, bool RING>
+__global__ __launch_bounds__(64 * BY) void batch_kernel(const BatchDesc *__restrict__ d, int nprob, int nitems,
+                                                        unsigned long long *ctr, unsigned long long base, int step) {
+  using ST = SweepTile<float, C, 0, BY, RY, true, false, RING, 2>;
+  __shared__ typename ST::Lds lds;
+  __shared__ int sclaim[2];
+  const bool leader = threadIdx.x == 0 && threadIdx.y == 0;
+  for (;;) {
+    if (leader) {
+      const int it = (int)(atomicAdd(ctr, 1ull) - base);
+      int p = 0;
+      if (it < nitems)
+        while (p + 1 < nprob && d[p + 1].item0 <= it) ++p;
+      sclaim[0] = it;
+      sclaim[1] = p;
+    }
+    __syncthreads();
+    const int it = __builtin_amdgcn_readfirstlane(sclaim[0]);
+    const int p = __builtin_amdgcn_readfirstlane(sclaim[1]);
+    if (it >= nitems) break;
+    const BatchDesc D = uniform_load(d + p);
+    SweepArgs<float, C> a{};
+    a.u = D.u;
+    a.v = D.v;
+    a.roc2 = (const C *)D.roc2;
+    for (int r = 0; r < 5; ++r) a.c[r] = D.c[r];
+    a.csum = D.csum;
+    a.g.sy = D.sy;
+    a.g.sz = D.sz;
+    a.g.xb = a.g.yb = a.g.zb = 4;
+    a.g.xe = D.xe;
+    a.g.ye = D.ye;
+    a.g.ze = D.ze;
+    a.tiles_x = D.tiles_x;
+    a.tiles_y = D.tiles_y;
+    a.nzc = D.nzc;
+    a.zchunk = D.zchunk;
+    const bool src = D.src && step < D.nsrc;
+    a.src = src ? D.src + step : nullptr;
+    a.src_k = src ? D.src_k : -1;
+    a.src_off = D.src_off;
+    const int local = it - D.item0;
+    const int txy = D.tiles_x * D.tiles_y;
+    const int zc = local / txy, rem = local - zc * txy;
+    const int ty = rem / D.tiles_x;
+    ST::run(a, rem - ty * D.tiles_x, ty, zc, lds);
+    __syncthreads();  // LDS and the claim slots are reused by the next item
+  }
+}
+
+typedef void (*batch_fn)(const BatchDesc *, int, int, unsigned long long *, unsigned long long, int);
+
+struct BatchVariant {
+  int by, ry;
+  bool ring;
+  batch_fn f32, bf16;
+};
+
+#define GR_BV(BY, RY, RING) \
+  { BY, RY, RING, &batch_kernel<float, BY, RY, RING>, &batch_kernel<__hip_bfloat16, BY, RY, RING> }
+const BatchVariant kVariants[] = {GR_BV(8, 4, false), GR_BV(8, 4, true), GR_BV(4, 4, false), GR_BV(8, 2, true),
+                                  GR_BV(4, 4, true)};
+#undef GR_BV
+constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
+
+}  // namespace
+}  // namespace gr
+
+using namespace gr;
+
+struct gr_batch {
+  std::vector<gr_handle *> probs;  // in the caller's order
+  std::vector<int> order;          // work order: largest grid first
+  int variant = 0, zchunk = 0;
+  int nitems = 0, grid = 0;
+  int occupancy[kNumVariants] = {};
+  int ncu = 0;
+  BatchDesc *ddesc = nullptr;  // [4][n]: step parity 0/1 of the problem fields, then of the tuning spares
+  std::vector<BatchDesc> hdesc;
+  unsigned long long *ctr = nullptr;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  bool bf16 = false;
+  // cached step-loop graph
+  hipGraphExec_t gexec = nullptr;
+  int g_napply = -1, g_variant = -1, g_zchunk = -1, g_table = -1;
+  double last_ms = 0;
+  int last_launches = 0;
+  long long last_updates = 0;
+};
+
+namespace {
+
+int items_of(const Problem &P, int zchunk, int ty_cells, int *tx, int *ty, int *nzc, int *zc) {
+  *tx = (P.nx + 63) / 64;
+  *ty = (P.ny + ty_cells - 1) / ty_cells;
+  *zc = (zchunk <= 0 || zchunk >= P.nz) ? P.nz : zchunk;
+  *nzc = (P.nz + *zc - 1) / *zc;
+  return *tx * *ty * *nzc;
+}
+
+// Fill the four descriptor tables: [0] / [1] advance the problems' fields from
+// their current parity (table 1 for odd steps of a call), [2] / [3] ping-pong
+// between the two spare fields (tuning runs; no source).
+int build_tables(gr_batch *b, bool spares) {
+  const int n = (int)b->probs.size();
+  const BatchVariant &V = kVariants[b->variant];
+  b->hdesc.assign(4 * (size_t)n, BatchDesc{});
+  int item = 0;
+  for (int oi = 0; oi < n; ++oi) {
+    const int i = b->order[oi];
+    gr_handle *h = b->probs[i];
+    const Problem &P = gr_internal_problem(h);
+    BatchDesc D{};
+    D.sy = P.pitch;
+    D.sz = (long long)P.pitch * P.rows;
+    D.xe = P.nx + 4;
+    D.ye = P.ny + 4;
+    D.ze = P.nz + 4;
+    const int cnt = items_of(P, b->zchunk, V.by * V.ry, &D.tiles_x, &D.tiles_y, &D.nzc, &D.zchunk);
+    D.item0 = item;
+    item += cnt;
+    D.roc2 = P.roc2;
+    for (int r = 0; r < 5; ++r) D.c[r] = (float)P.c[r];
+    D.csum = (float)(P.c[0] + 6.0 * (P.c[1] + P.c[2]));
+    const int par = gr_internal_parity(h);
+    int nsrc = 0;
+    long long sidx = -1;
+    const float *src = (const float *)gr_internal_source(h, &nsrc, &sidx);
+    const long long done = gr_internal_napplied(h);
+    const bool has_src = src && sidx >= 0 && done < nsrc;
+    for (int q = 0; q < 2; ++q) {
+      BatchDesc E = D;
+      const int w = par ^ q;  // logical field written at this step parity
+      E.u = (float *)gr_internal_field(h, w);
+      E.v = (const float *)gr_internal_field(h, 1 - w);
+      E.src = has_src ? src + done : nullptr;
+      E.nsrc = has_src ? (int)(nsrc - done) : 0;
+      E.src_k = has_src ? (int)(sidx / D.sz) : -1;
+      E.src_off = has_src ? (int)(sidx % D.sz) : 0;
+      b->hdesc[(size_t)q * n + oi] = E;
+      if (spares) {
+        BatchDesc S = D;
+        S.u = (float *)gr_internal_spare(h, q);
+        S.v = (const float *)gr_internal_spare(h, 1 - q);
+        if (!S.u || !S.v) return set_error(GR_ENOMEM, "gr_batch: cannot allocate tuning spares");
+        S.src = nullptr;
+        S.src_k = -1;
+        b->hdesc[(size_t)(2 + q) * n + oi] = S;
+      }
+    }
+  }
+  b->nitems = item;
+  b->grid = std::max(1, std::min(item, b->ncu * std::max(1, b->occupancy[b->variant])));
+  const size_t bytes = sizeof(BatchDesc) * (spares ? 4 : 2) * (size_t)n;
+  hipError_t e = hipMemcpyAsync(b->ddesc, b->hdesc.data(), bytes, hipMemcpyHostToDevice, b->stream);
+  if (e != hipSuccess) return set_errorf(GR_EHIP, "gr_batch: descriptor copy: %s", hipGetErrorString(e));
+  return GR_OK;
+}
+
+hipError_t enqueue_steps(gr_batch *b, int table, int napply, hipStream_t st) {
+  const int n = (int)b->probs.size();
+  const BatchVariant &V = kVariants[b->variant];
+  const batch_fn f = b->bf16 ? V.bf16 : V.f32;
+  hipLaunchKernelGGL(batch_reset_kernel, dim3(1), dim3(64), 0, st, b->ctr);
+  unsigned long long base = 0;
+  for (int s = 0; s < napply; ++s) {
+    const BatchDesc *tab = b->ddesc + (size_t)(table + (s & 1)) * n;
+    hipLaunchKernelGGL(f, dim3(b->grid), dim3(64, V.by), 0, st, tab, n, b->nitems, b->ctr, base, s);
+    base += (unsigned long long)(b->nitems + b->grid);
+  }
+  return hipGetLastError();
+}
+
+void drop_graph(gr_batch *b) {
+  if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+  b->gexec = nullptr;
+  b->g_napply = b->g_variant = b->g_zchunk = b->g_table = -1;
+}
+
+// Run napply steps from descriptor table `table` (0: the problems, 2: spares).
+int run_steps(gr_batch *b, int table, int napply, bool graph) {
+  hipError_t e = hipSuccess;
+  if (graph && napply > 0) {
+    if (!b->gexec || b->g_napply != napply || b->g_variant != b->variant || b->g_zchunk != b->zchunk ||
+        b->g_table != table) {
+      drop_graph(b);
+      hipGraph_t g = nullptr;
+      e = hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal);
+      if (e == hipSuccess) {
+        const hipError_t e2 = enqueue_steps(b, table, napply, b->stream);
+        e = hipStreamEndCapture(b->stream, &g);
+        if (e == hipSuccess) e = e2;
+      }
+      if (e == hipSuccess) e = hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0);
+      if (g) (void)hipGraphDestroy(g);
+      if (e != hipSuccess) {
+        drop_graph(b);
+        return set_errorf(GR_EHIP, "gr_batch: graph capture: %s", hipGetErrorString(e));
+      }
+      b->g_napply = napply;
+      b->g_variant = b->variant;
+      b->g_zchunk = b->zchunk;
+      b->g_table = table;
+    }
+    e = hipEventRecord(b->ev0, b->stream);
+    if (e == hipSuccess) e = hipGraphLaunch(b->gexec, b->stream);
+  } else {
+    e = hipEventRecord(b->ev0, b->stream);
+    if (e == hipSuccess) e = enqueue_steps(b, table, napply, b->stream);
+  }
+  if (e == hipSuccess) e = hipEventRecord(b->ev1, b->stream);
+  if (e == hipSuccess) e = hipEventSynchronize(b->ev1);
+  if (e != hipSuccess) return set_errorf(GR_EHIP, "gr_batch: step launch: %s", hipGetErrorString(e));
+  float ms = 0.f;
+  (void)hipEventElapsedTime(&ms, b->ev0, b->ev1);
+  b->last_ms = ms;
+  b->last_launches = napply;
+  long long up = 0;
+  for (gr_handle *h : b->probs) {
+    const Problem &P = gr_internal_problem(h);
+    up += (long long)P.nx * P.ny * P.nz;
+  }
+  b->last_updates = up * napply;
+  return GR_OK;
+}
+
+int sync_problems(gr_batch *b) {
+  for (gr_handle *h : b->probs) {
+    const hipError_t e = hipStreamSynchronize(gr_internal_stream(h));
+    if (e != hipSuccess) return set_errorf(GR_EHIP, "gr_batch: %s", hipGetErrorString(e));
+  }
+  return GR_OK;
+}
+
+}  // namespace
+
+extern "C" int gr_batch_destroy(gr_batch *b) {
+  if (!b) return GR_OK;
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  drop_graph(b);
+  for (gr_handle *h : b->probs) gr_destroy(h);
+  if (b->ddesc) (void)hipFree(b->ddesc);
+  if (b->ctr) (void)hipFree(b->ctr);
+  if (b->ev0) (void)hipEventDestroy(b->ev0);
+  if (b->ev1) (void)hipEventDestroy(b->ev1);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+  return GR_OK;
+}
+
+extern "C" int gr_batch_create(const gr_desc *descs, int n, gr_batch **out) {
+  if (!descs || !out) return set_error(GR_EINVAL, "gr_batch_create: null argument");
+  *out = nullptr;
+  if (n < 1 || n > kMaxBatch) return set_errorf(GR_EINVAL, "gr_batch_create: need 1..%d problems", kMaxBatch);
+  const int flags0 = descs[0].flags;
+  for (int i = 0; i < n; ++i) {
+    const gr_desc &d = descs[i];
+    if (d.kernel != 0 || d.dtype != GR_F32 || !(d.flags & GR_FAST) || (d.flags & GR_ROC2_SCALAR))
+      return set_errorf(GR_EUNSUPPORTED,
+                        "gr_batch_create: problem %d: batches hold kernel 0, GR_F32 | GR_FAST problems with a "
+                        "roc2 field", i);
+    if (d.flags != flags0) return set_error(GR_EINVAL, "gr_batch_create: all problems need the same flags");
+  }
+  gr_batch *b = new gr_batch();
+  b->bf16 = (flags0 & GR_COEF_BF16) != 0;
+  int rc = GR_OK;
+  for (int i = 0; i < n && rc == GR_OK; ++i) {
+    gr_handle *h = nullptr;
+    rc = gr_create(&descs[i], &h);
+    if (rc == GR_OK) b->probs.push_back(h);
+  }
+  hipError_t e = hipSuccess;
+  if (rc == GR_OK) {
+    int dev = 0;
+    e = hipMalloc(&b->ddesc, sizeof(BatchDesc) * 4 * (size_t)n);
+    if (e == hipSuccess) e = hipMalloc(&b->ctr, sizeof(unsigned long long));
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreate(&b->ev0);
+    if (e == hipSuccess) e = hipEventCreate(&b->ev1);
+    if (e == hipSuccess) e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&b->ncu, hipDeviceAttributeMultiprocessorCount, dev);
+    for (int v = 0; v < kNumVariants && e == hipSuccess; ++v) {
+      const BatchVariant &V = kVariants[v];
+      e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+          &b->occupancy[v], reinterpret_cast<const void *>(b->bf16 ? V.bf16 : V.f32), 64 * V.by, 0);
+    }
+    if (e != hipSuccess) rc = set_errorf(GR_EHIP, "gr_batch_create: %s", hipGetErrorString(e));
+  }
+  if (rc != GR_OK) {
+    gr_batch_destroy(b);
+    return rc;
+  }
+  b->order.resize(n);
+  std::iota(b->order.begin(), b->order.end(), 0);
+  std::stable_sort(b->order.begin(), b->order.end(), [&](int x, int y) {
+    const Problem &A = gr_internal_problem(b->probs[x]), &B = gr_internal_problem(b->probs[y]);
+    return (long long)A.nx * A.ny * A.nz > (long long)B.nx * B.ny * B.nz;
+  });
+  *out = b;
+  return GR_OK;
+}
+
+extern "C" int gr_batch_problem(gr_batch *b, int i, gr_handle **h) {
+  if (!b || !h) return set_error(GR_EINVAL, "gr_batch_problem: null argument");
+  if (i < 0 || i >= (int)b->probs.size()) return set_errorf(GR_EINVAL, "gr_batch_problem: no problem %d", i);
+  *h = b->probs[i];
+  return GR_OK;
+}
+
+extern "C" int gr_batch_set_config(gr_batch *b, int variant, int zchunk) {
+  if (!b) return set_error(GR_EINVAL, "gr_batch_set_config: null batch");
+  if (variant >= kNumVariants)
+    return set_errorf(GR_EINVAL, "gr_batch_set_config: variant %d (0..%d)", variant, kNumVariants - 1);
+  if (variant >= 0) b->variant = variant;
+  if (zchunk >= 0) b->zchunk = zchunk;
+  return GR_OK;
+}
+
+extern "C" int gr_batch_step(gr_batch *b, int napply, int use_graph) {
+  if (!b) return set_error(GR_EINVAL, "gr_batch_step: null batch");
+  if (napply < 0) return set_error(GR_EINVAL, "gr_batch_step: negative step count");
+  int rc = sync_problems(b);
+  if (rc == GR_OK) rc = build_tables(b, false);
+  if (rc == GR_OK) rc = run_steps(b, 0, napply, use_graph != 0);
+  if (rc != GR_OK) return rc;
+  for (gr_handle *h : b->probs) gr_internal_advance(h, napply, b->last_ms);
+  return GR_OK;
+}
+
+extern "C" int gr_batch_autotune(gr_batch *b, int steps, int use_graph, int *variant, int *zchunk) {
+  if (!b) return set_error(GR_EINVAL, "gr_batch_autotune: null batch");
+  if (steps < 1) steps = 4;
+  int rc = sync_problems(b);
+  if (rc) return rc;
+  const int zcands[] = {0, 256, 128, 64};
+  double best = 1e30;
+  int bv = b->variant, bz = b->zchunk;
+  for (int v = 0; v < kNumVariants; ++v) {
+    if (b->occupancy[v] < 1) continue;
+    for (int zc : zcands) {
+      b->variant = v;
+      b->zchunk = zc;
+      if ((rc = build_tables(b, true))) return rc;
+      if ((rc = run_steps(b, 2, 1, false))) return rc;  // warm-up
+      if ((rc = run_steps(b, 2, steps, use_graph != 0))) return rc;
+      if (b->last_ms < best) {
+        best = b->last_ms;
+        bv = v;
+        bz = zc;
+      }
+    }
+  }
+  drop_graph(b);
+  b->variant = bv;
+  b->zchunk = bz;
+  if (variant) *variant = bv;
+  if (zchunk) *zchunk = bz;
+  return GR_OK;
+}
+
+extern "C" int gr_batch_get_stats(gr_batch *b, gr_batch_stats *out) {
+  if (!b || !out) return set_error(GR_EINVAL, "gr_batch_get_stats: null argument");
+  gr_batch_stats s{};
+  s.last_ms = b->last_ms;
+  s.last_updates = b->last_updates;
+  s.last_launches = b->last_launches;
+  s.nproblems = (int)b->probs.size();
+  s.items = b->nitems;
+  s.grid = b->grid;
+  s.variant = b->variant;
+  s.zchunk = b->zchunk;
+  s.tile_x = 64;
+  s.tile_y = kVariants[b->variant].by * kVariants[b->variant].ry;
+  s.nvariants = kNumVariants;
+  *out = s;
+  return GR_OK;
+}

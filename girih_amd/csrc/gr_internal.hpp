@@ -31,6 +31,7 @@ void tile_of_variant(int kid, int dtype, int variant, int *tx, int *ty);
 
 size_t elem_size(int dtype);
 int set_error(int code, const char *msg);
+int set_errorf(int code, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
 hipError_t sweep_fields(const Problem &P, void *u, const void *v, const void *src, long long src_idx,
                         const LaunchCfg &lc, hipStream_t st);
 hipError_t tblock_pass(const Problem &P, int tl, const void *X, const void *Y, void *ZO, void *ZN, const void *src,
@@ -39,3 +40,14 @@ int tb_depths(int kid, int dtype, int *out, int max);
 hipError_t sweep_box(const Problem &P, const Geom &g, void *u, const void *v, const LaunchCfg &lc, hipStream_t st);
 
 }  // namespace gr
+
+// Handle internals for the batched stepper (gr_batch.hip).
+struct gr_handle;
+const gr::Problem &gr_internal_problem(const gr_handle *h);
+void *gr_internal_field(const gr_handle *h, int logical);  // 0 U1, 1 U2 (x offset applied)
+void *gr_internal_spare(gr_handle *h, int which);          // spare field 0/1, allocated on demand (or null)
+int gr_internal_parity(const gr_handle *h);
+long long gr_internal_napplied(const gr_handle *h);
+const void *gr_internal_source(const gr_handle *h, int *n, long long *idx);
+hipStream_t gr_internal_stream(const gr_handle *h);
+void gr_internal_advance(gr_handle *h, int napply, double ms);

@@ -50,6 +50,13 @@ int fail(int code, const char *fmt, ...) {
 namespace gr {
 
 int set_error(int code, const char *msg) { return fail(code, "%s", msg); }
+int set_errorf(int code, const char *fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+  return code;
+}
 
 size_t elem_size(int dtype) { return dtype == GR_F64 ? 8 : 4; }
 
@@ -997,4 +1004,29 @@ extern "C" int gr_compare_fields(gr_handle *a, int fa, gr_handle *b, int fb, dou
     out[3] = std::max(out[3], h[4 * i + 3]);
   }
   return GR_OK;
+}
+
+// ------------------------------------------------------------------ internals for gr_batch.hip
+const gr::Problem &gr_internal_problem(const gr_handle *h) { return h->P; }
+void *gr_internal_field(const gr_handle *h, int logical) { return ubase(h, logical); }
+void *gr_internal_spare(gr_handle *h, int which) {
+  if (ensure_spares(h) != GR_OK) return nullptr;
+  int sp[2];
+  spare_ids(h, sp);
+  return field_base(h, h->fbuf[sp[which & 1]]);
+}
+int gr_internal_parity(const gr_handle *h) { return h->parity; }
+long long gr_internal_napplied(const gr_handle *h) { return h->napplied; }
+const void *gr_internal_source(const gr_handle *h, int *n, long long *idx) {
+  *n = h->nsrc;
+  *idx = h->src_idx;
+  return h->src;
+}
+hipStream_t gr_internal_stream(const gr_handle *h) { return h->stream; }
+void gr_internal_advance(gr_handle *h, int napply, double ms) {
+  h->parity ^= napply & 1;
+  h->napplied += napply;
+  h->stats.last_ms = ms;
+  h->stats.last_updates = (int64_t)napply * h->P.nx * h->P.ny * h->P.nz;
+  h->stats.last_launches = napply;
 }

@@ -22,8 +22,9 @@
 //   * reference order (default): the association and operand order of girih's
 //     FUNC_BODY, compiled with -ffp-contract=off, so FP64 and FP32 results are
 //     bit-identical to girih's serial reference;
-//   * FAST (FP32, kernel 0): per-radius neighbour sums, FMA accumulation and a
-//     TwoSum-compensated 2u^n - u^(n-1) update (only the final store rounds).
+//   * FAST (FP32, kernel 0): per-radius neighbour sums (radii 1-2 as
+//     differences from the centre), FMA accumulation and the increment form
+//     u^n + ((u^n - u^(n-1)) + roc2*lap) of the leapfrog update.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <hip/hip_bf16.h>
@@ -180,6 +181,9 @@ struct Tile {
   static constexpr int RY = (KID == 4) ? 2 : 4;
 };
 
+template <typename T, typename C, int KID, int BY, int RY, bool FAST, bool ROCS, bool RING, int ND>
+struct SweepTile;
+
 // MINW > 0 asks the compiler for at least MINW waves per SIMD (caps VGPRs at 512/MINW).
 // RING: the LDS holds a ring of R+2 tile planes and the halo ring of a plane is
 // fetched in the same step as the interior of that plane (R+1 planes ahead), when
@@ -188,22 +192,37 @@ struct Tile {
 template <typename T, typename C, int KID, int BY, int RY, bool FAST, bool ROCS, int MINW, bool RING, int ND = 2>
 __global__ __launch_bounds__(64 * BY) __attribute__((amdgpu_waves_per_eu(MINW > 0 ? MINW : 1)))
 void sweep_kernel(const SweepArgs<T, C> a) {
-  constexpr int R = kradius(KID);
-  constexpr int Q = 2 * R + 1;
-  constexpr int TY = BY * RY;
-  constexpr int NT = 64 * BY;
-  constexpr int NC = kncoef(KID);
-  using H = Halo<R, TY>;
-  constexpr int LX = H::LX, LY = H::LY;
-  constexpr int HS = (H::N + NT - 1) / NT;  // halo slots per lane
-  constexpr int S = RING ? R + 2 : 2;        // LDS tile planes
-  __shared__ T lds[S][LY][LX];
-
+  using ST = SweepTile<T, C, KID, BY, RY, FAST, ROCS, RING, ND>;
+  __shared__ typename ST::Lds lds;
   const int nb = a.tiles_x * a.tiles_y * a.nzc;
   const int lb = a.xcd_order ? xcd_remap(blockIdx.x, nb) : (int)blockIdx.x;
   const int tile_x = lb % a.tiles_x;
   const int tile_y = (lb / a.tiles_x) % a.tiles_y;
   const int zc = lb / (a.tiles_x * a.tiles_y);
+  ST::run(a, tile_x, tile_y, zc, lds);
+}
+
+// One tile of one sweep: the z-streaming body shared by the single-grid kernel
+// and the batched persistent kernel.
+template <typename T, typename C, int KID, int BY, int RY, bool FAST, bool ROCS, bool RING, int ND>
+struct SweepTile {
+  static constexpr int R = kradius(KID);
+  static constexpr int TY = BY * RY;
+  static constexpr int LX = Halo<R, TY>::LX, LY = Halo<R, TY>::LY;
+  static constexpr int S = RING ? R + 2 : 2;  // LDS tile planes
+  typedef T Lds[S][LY][LX];
+  __device__ static __forceinline__ void run(const SweepArgs<T, C> &a, int tile_x, int tile_y, int zc, Lds &lds);
+};
+
+template <typename T, typename C, int KID, int BY, int RY, bool FAST, bool ROCS, bool RING, int ND>
+__device__ __forceinline__ void SweepTile<T, C, KID, BY, RY, FAST, ROCS, RING, ND>::run(const SweepArgs<T, C> &a,
+                                                                                   int tile_x, int tile_y, int zc,
+                                                                                   Lds &lds) {
+  constexpr int Q = 2 * R + 1;
+  constexpr int NT = 64 * BY;
+  constexpr int NC = kncoef(KID);
+  using H = Halo<R, TY>;
+  constexpr int HS = (H::N + NT - 1) / NT;  // halo slots per lane
 
   const int tx = threadIdx.x, ty = threadIdx.y;
   const int tid = ty * 64 + tx;

@@ -50,9 +50,26 @@ class Stats(ctypes.Structure):
                 ("x_offset", ctypes.c_int32)]
 
 
+class VModel(ctypes.Structure):
+    _fields_ = [("seed", ctypes.c_uint64), ("nlayers", ctypes.c_int32), ("nblobs", ctypes.c_int32),
+                ("vmin", ctypes.c_double), ("vmax", ctypes.c_double), ("pert", ctypes.c_double),
+                ("dt_over_h", ctypes.c_double), ("anomaly", ctypes.c_double),
+                ("anomaly_center", ctypes.c_double * 3), ("anomaly_sigma", ctypes.c_double),
+                ("round_bf16", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+
+
+class BatchStats(ctypes.Structure):
+    _fields_ = [("last_ms", ctypes.c_double), ("last_updates", ctypes.c_int64), ("last_launches", ctypes.c_int32),
+                ("nproblems", ctypes.c_int32), ("items", ctypes.c_int32), ("grid", ctypes.c_int32),
+                ("variant", ctypes.c_int32), ("zchunk", ctypes.c_int32), ("tile_x", ctypes.c_int32),
+                ("tile_y", ctypes.c_int32), ("nvariants", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+
+
 EXPORTS = ["gr_create", "gr_destroy", "gr_last_error", "gr_upload", "gr_download", "gr_step", "gr_run_ts",
            "gr_set_source", "gr_get_stats", "gr_mwd_round_nt", "gr_sweep_dev", "gr_field_ptr", "gr_init_fields",
-           "gr_init_coef_uniform", "gr_compare_fields", "gr_set_config", "gr_autotune"]
+           "gr_init_coef_uniform", "gr_compare_fields", "gr_set_config", "gr_autotune", "gr_init_roc2_model",
+           "gr_download_roc2", "gr_batch_create", "gr_batch_destroy", "gr_batch_problem", "gr_batch_set_config",
+           "gr_batch_step", "gr_batch_autotune", "gr_batch_get_stats"]
 
 _lib = None
 
@@ -91,6 +108,15 @@ def lib():
             "gr_compare_fields": ([P, I, P, I, ctypes.POINTER(ctypes.c_double)], I),
             "gr_set_config": ([P, I, I, I], I),
             "gr_autotune": ([P, I, ctypes.POINTER(Stats)], I),
+            "gr_init_roc2_model": ([P, ctypes.POINTER(VModel)], I),
+            "gr_download_roc2": ([P, IP, P], I),
+            "gr_batch_create": ([ctypes.POINTER(Desc), I, ctypes.POINTER(P)], I),
+            "gr_batch_destroy": ([P], I),
+            "gr_batch_problem": ([P, I, ctypes.POINTER(P)], I),
+            "gr_batch_set_config": ([P, I, I], I),
+            "gr_batch_step": ([P, I, I], I),
+            "gr_batch_autotune": ([P, I, I, IP, IP], I),
+            "gr_batch_get_stats": ([P, ctypes.POINTER(BatchStats)], I),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -130,26 +156,49 @@ def _ptr(a):
     return a.ctypes.data_as(ctypes.c_void_p)
 
 
+def make_desc(kernel, nx, ny, nz, dtype=GR_F64, flags=0, t_block=1, roc2=0.0, coef=None):
+    d = Desc()
+    d.kernel, d.dtype, d.nx, d.ny, d.nz = kernel, dtype, nx, ny, nz
+    d.flags, d.t_block, d.roc2 = flags, t_block, roc2
+    c = G_COEF if coef is None else coef
+    for i, x in enumerate(list(c)[:11]):
+        d.coef[i] = float(x)
+    return d
+
+
+def vmodel(seed, dt_over_h, nlayers=8, nblobs=6, vmin=1500.0, vmax=4500.0, pert=0.05, anomaly=0.0,
+           anomaly_center=(0.0, 0.0, 0.0), anomaly_sigma=1.0, round_bf16=False):
+    """A layered velocity model (gr_vmodel): nlayers layers along z from vmin to
+    vmax m/s, random interfaces and a +-pert Gaussian-blob perturbation."""
+    m = VModel()
+    m.seed, m.nlayers, m.nblobs = seed, nlayers, nblobs
+    m.vmin, m.vmax, m.pert, m.dt_over_h = vmin, vmax, pert, dt_over_h
+    m.anomaly, m.anomaly_sigma = anomaly, anomaly_sigma
+    for d in range(3):
+        m.anomaly_center[d] = float(anomaly_center[d])
+    m.round_bf16 = 1 if round_bf16 else 0
+    return m
+
+
 class Stepper:
     """One stencil problem resident in HBM (a girih time-stepper target)."""
 
-    def __init__(self, kernel, nx, ny, nz, dtype=GR_F64, flags=0, t_block=1, roc2=0.0, coef=None):
+    def __init__(self, kernel, nx, ny, nz, dtype=GR_F64, flags=0, t_block=1, roc2=0.0, coef=None, _handle=None):
         self.kernel, self.nx, self.ny, self.nz = kernel, nx, ny, nz
         self.dtype, self.flags = dtype, flags
         self.R = RADIUS.get(kernel, 1)
-        d = Desc()
-        d.kernel, d.dtype, d.nx, d.ny, d.nz = kernel, dtype, nx, ny, nz
-        d.flags, d.t_block, d.roc2 = flags, t_block, roc2
-        c = G_COEF if coef is None else coef
-        for i, x in enumerate(list(c)[:11]):
-            d.coef[i] = float(x)
+        self._owned = _handle is None
+        if _handle is not None:  # a problem owned by a Batch
+            self._h = _handle
+            return
+        d = make_desc(kernel, nx, ny, nz, dtype, flags, t_block, roc2, coef)
         self._h = ctypes.c_void_p()
         check(lib().gr_create(ctypes.byref(d), ctypes.byref(self._h)))
 
     def close(self):
-        if self._h:
+        if self._h and self._owned:
             lib().gr_destroy(self._h)
-            self._h = ctypes.c_void_p()
+        self._h = ctypes.c_void_p()
 
     def __del__(self):
         try:
@@ -230,6 +279,77 @@ class Stepper:
         p = ctypes.c_void_p()
         check(lib().gr_field_ptr(self._h, int(which), ctypes.byref(p)))
         return p.value
+
+    def init_roc2_model(self, model):
+        check(lib().gr_init_roc2_model(self._h, ctypes.byref(model)))
+
+    def download_roc2(self, shape=None):
+        if shape is None:
+            nnx, nny, nnz = ldomain_shape(self.kernel, self.nx, self.ny, self.nz)
+            shape = (nnz, nny, nnx)
+        U3 = np.zeros(shape, dtype=self.np_dtype)
+        check(lib().gr_download_roc2(self._h, self._ld(U3), _ptr(U3)))
+        return U3
+
+
+class Batch:
+    """Independent kernel-0 problems of uneven size advanced together, one
+    persistent work-queue launch per time step (gr_batch_*)."""
+
+    def __init__(self, sizes, flags=GR_FAST | GR_COEF_BF16, coef=None):
+        self.sizes = [tuple(int(x) for x in s) for s in sizes]
+        descs = (Desc * len(self.sizes))()
+        for i, (nx, ny, nz) in enumerate(self.sizes):
+            descs[i] = make_desc(0, nx, ny, nz, GR_F32, flags, 1, 0.0, coef)
+        self.flags = flags
+        self._b = ctypes.c_void_p()
+        check(lib().gr_batch_create(descs, len(self.sizes), ctypes.byref(self._b)))
+        self.problems = []
+        for i, (nx, ny, nz) in enumerate(self.sizes):
+            h = ctypes.c_void_p()
+            check(lib().gr_batch_problem(self._b, i, ctypes.byref(h)))
+            self.problems.append(Stepper(0, nx, ny, nz, GR_F32, flags, _handle=h))
+
+    def close(self):
+        if self._b:
+            for p in self.problems:
+                p.close()
+            lib().gr_batch_destroy(self._b)
+            self._b = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __len__(self):
+        return len(self.problems)
+
+    def __getitem__(self, i):
+        return self.problems[i]
+
+    def step(self, napply, use_graph=False):
+        check(lib().gr_batch_step(self._b, int(napply), 1 if use_graph else 0))
+
+    def set_config(self, variant=-1, zchunk=-1):
+        check(lib().gr_batch_set_config(self._b, int(variant), int(zchunk)))
+
+    def autotune(self, steps=4, use_graph=False):
+        v, z = ctypes.c_int(0), ctypes.c_int(0)
+        check(lib().gr_batch_autotune(self._b, int(steps), 1 if use_graph else 0, ctypes.byref(v), ctypes.byref(z)))
+        return v.value, z.value
+
+    def stats(self):
+        s = BatchStats()
+        check(lib().gr_batch_get_stats(self._b, ctypes.byref(s)))
+        return {f: getattr(s, f) for f, _ in BatchStats._fields_}
 
 
 def sweep_dev(kernel, dtype, shape_xyz, box, coef, u_ptr, v_ptr, roc2_ptr=None):

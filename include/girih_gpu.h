@@ -37,9 +37,11 @@ extern "C" {
 
 /* gr_desc.flags */
 #define GR_ROC2_SCALAR 0x1  /* kernel 0: roc2 is the constant gr_desc.roc2, no U3 array */
-#define GR_FAST 0x2         /* GR_F32 only: neighbour pairs summed per radius before scaling,
-                               FMA accumulation and a compensated (TwoSum) 2u^n-u^(n-1)
-                               update; not bit-identical to girih's operand order */
+#define GR_FAST 0x2         /* GR_F32 kernel 0 only: neighbour pairs summed per radius before
+                               scaling (radii 1-2 as differences from the centre), FMA
+                               accumulation and the increment form u^n + ((u^n - u^(n-1))
+                               + roc2*lap) of the leapfrog update; not bit-identical to
+                               girih's operand order */
 #define GR_COEF_BF16 0x4    /* roc2 / coefficient arrays stored in HBM as bfloat16
                                (computed in the field precision) */
 
@@ -158,6 +160,66 @@ int gr_autotune(gr_handle *h, int steps, gr_stats *best);
  * precisions may differ).  out = { sum (a-b)^2, sum b^2, max |a-b|, max |b| }.
  * b may equal a (then fa != fb compares the two fields of one handle). */
 int gr_compare_fields(gr_handle *a, int fa, gr_handle *b, int fb, double out[4]);
+
+/* ---- Velocity models for kernel 0 (roc2 = (v*dt/h)^2, girih's U3 array) ----
+ * No counterpart in girih, which fills U3 with its closed-form ramp
+ * (utils.c:605-697).  A layered earth: nlayers layers along z, speeds rising
+ * linearly from vmin (top) to vmax (bottom), interface depths drawn from the
+ * seed; speeds scaled by 1 + pert * clamp(sum of nblobs random signed Gaussian
+ * blobs, -1, 1) and by 1 + anomaly * exp(-|x - c|^2 / (2 sigma^2)).  The model is
+ * a pure function of the struct and the grid size, whatever the field type. */
+typedef struct gr_vmodel {
+  uint64_t seed;
+  int32_t nlayers;            /* 1..16 */
+  int32_t nblobs;             /* 0..16 */
+  double vmin, vmax;          /* m/s */
+  double pert;                /* relative bound of the blob perturbation, [0, 1) */
+  double dt_over_h;           /* time step / grid spacing (s/m) */
+  double anomaly;             /* relative amplitude of the centred anomaly (0 = none) */
+  double anomaly_center[3];   /* interior-relative cells (i-R, j-R, k-R) */
+  double anomaly_sigma;       /* cells */
+  int32_t round_bf16;         /* round roc2 to bfloat16 values in any storage */
+  int32_t reserved;
+} gr_vmodel;
+int gr_init_roc2_model(gr_handle *h, const gr_vmodel *model);
+/* Copy roc2 to a girih-layout host array of the field type (bf16 storage widened). */
+int gr_download_roc2(gr_handle *h, const int ldomain_shape[3], void *U3);
+
+/* ---- Batched problems (no counterpart in girih, which steps one grid) ----
+ * n independent kernel-0 problems (GR_F32 | GR_FAST, roc2 field, optionally
+ * GR_COEF_BF16; same flags for all) of any sizes advance together: each time step
+ * is ONE persistent launch whose workgroups claim (problem, xy tile, z chunk)
+ * items from a device work queue, largest problem first.  Every problem keeps
+ * its own handle (gr_batch_problem) for uploads, models, sources, downloads and
+ * comparisons; its results equal gr_step's on the same handle. */
+typedef struct gr_batch gr_batch;
+typedef struct gr_batch_stats {
+  double last_ms;           /* device time of the last gr_batch_step / tuning run */
+  int64_t last_updates;     /* lattice-site updates of that call (all problems) */
+  int32_t last_launches;    /* step launches of that call */
+  int32_t nproblems;
+  int32_t items;            /* work items per step */
+  int32_t grid;             /* persistent workgroups per launch */
+  int32_t variant, zchunk;  /* tile variant and z planes per item (0 = whole grid) */
+  int32_t tile_x, tile_y;
+  int32_t nvariants;        /* tile variants available */
+  int32_t reserved;
+} gr_batch_stats;
+int gr_batch_create(const gr_desc *descs, int n, gr_batch **out);
+int gr_batch_destroy(gr_batch *b);
+/* Handle of problem i (owned by the batch: do not gr_destroy it). */
+int gr_batch_problem(gr_batch *b, int i, gr_handle **h);
+/* Tile variant (0..nvariants-1) and z planes per work item (0 = whole grid);
+ * values < 0 leave a setting unchanged.  Results never depend on them. */
+int gr_batch_set_config(gr_batch *b, int variant, int zchunk);
+/* Advance every problem by napply applications (girih's U1/U2 pairing per
+ * problem).  use_graph != 0 replays the step loop from a HIP graph (captured on
+ * first use, kept while napply and the configuration stay the same). */
+int gr_batch_step(gr_batch *b, int napply, int use_graph);
+/* Time every tile variant x z chunk for `steps` steps on scratch fields and keep
+ * the fastest; the problems' fields and parities are untouched. */
+int gr_batch_autotune(gr_batch *b, int steps, int use_graph, int *variant, int *zchunk);
+int gr_batch_get_stats(gr_batch *b, gr_batch_stats *out);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,172 @@
+"""Batched kernel-0 problems: one persistent work-queue launch per time step.
+
+* every problem of a batch ends bit-identical to the same problem stepped alone
+  with gr_step (same GR_FAST arithmetic, bf16 or FP32 roc2), whatever the tile
+  variant, z chunk, step parity or source;
+* graph replays of the step loop equal eager launches on every replay;
+* against the FP64 CPU oracle, with the roc2 the batch actually holds, the FP32
+  batch stays within a relative L2 error of 1e-5;
+* the layered velocity model has the stated layer speeds and bf16 rounding.
+"""
+import numpy as np
+import pytest
+
+import oracle as orc
+from girih_amd import gpu, physics
+
+pytestmark = pytest.mark.gpu
+
+SIZES = [(70, 40, 24), (64, 32, 16), (40, 72, 30), (130, 9, 11)]
+DT_OVER_H = 0.4 / 4725.0  # CFL 0.4 at the fastest perturbed speed
+
+
+def _model(seed, round_bf16=False, **kw):
+    return gpu.vmodel(seed, DT_OVER_H, round_bf16=round_bf16, **kw)
+
+
+def _init(batch, seed, src=None):
+    for i, p in enumerate(batch.problems):
+        p.init_fields(1, seed + i)
+        p.init_roc2_model(_model(seed + 100 + i))
+        if src is not None:
+            nx, ny, nz = batch.sizes[i]
+            p.set_source(4 + nx // 2, 4 + ny // 2, 4 + nz // 2, src)
+
+
+def _alone(size, flags, seed, i, napply, src=None, pre=0):
+    nx, ny, nz = size
+    with gpu.Stepper(0, nx, ny, nz, dtype=gpu.GR_F32, flags=flags, coef=physics.fd8_coef()) as st:
+        st.init_fields(1, seed + i)
+        st.init_roc2_model(_model(seed + 100 + i))
+        if src is not None:
+            st.set_source(4 + nx // 2, 4 + ny // 2, 4 + nz // 2, src)
+        if pre:
+            st.step(pre)
+        st.step(napply)
+        return st.download()
+
+
+@pytest.mark.parametrize("bf16", [True, False])
+@pytest.mark.parametrize("variant,zchunk", [(0, 0), (1, 8), (2, 0), (3, 16), (4, 7)])
+def test_batch_equals_single_problem_steps(bf16, variant, zchunk):
+    flags = gpu.GR_FAST | (gpu.GR_COEF_BF16 if bf16 else 0)
+    src = np.linspace(0.5, -0.2, 7)
+    print("batch", bf16, variant, zchunk, flags, flush=True)
+    with gpu.Batch(SIZES, flags=flags, coef=physics.fd8_coef()) as b:
+        _init(b, 11, src)
+        b.set_config(variant, zchunk)
+        b.step(5)
+        b.step(4)
+        st = b.stats()
+        assert st["variant"] == variant and st["zchunk"] == zchunk and st["last_launches"] == 4
+        got = [p.download() for p in b.problems]
+    for i, size in enumerate(SIZES):
+        U1, U2 = _alone(size, flags, 11, i, 9, src)
+        np.testing.assert_array_equal(got[i][0], U1, err_msg="problem %d U1" % i)
+        np.testing.assert_array_equal(got[i][1], U2, err_msg="problem %d U2" % i)
+
+
+def test_batch_mixed_parities_and_graph_replays():
+    """Problems at different step parities advance correctly, and a replayed step
+    graph equals eager launches on every replay (eager work in between)."""
+    flags = gpu.GR_FAST | gpu.GR_COEF_BF16
+    with gpu.Batch(SIZES, flags=flags, coef=physics.fd8_coef()) as eager, \
+            gpu.Batch(SIZES, flags=flags, coef=physics.fd8_coef()) as graph:
+        for b in (eager, graph):
+            _init(b, 5)
+            b.problems[1].step(1)  # problem 1 one application ahead (odd parity)
+            b.problems[3].step(3)
+        for rep in range(3):
+            print("replay", rep, flush=True)
+            eager.step(6)
+            graph.step(6, use_graph=True)
+            for i in range(len(SIZES)):
+                E = eager.problems[i].download()
+                G = graph.problems[i].download()
+                np.testing.assert_array_equal(E[0], G[0])
+                np.testing.assert_array_equal(E[1], G[1])
+        for i, pre in ((1, 1), (3, 3)):
+            U1, U2 = _alone(SIZES[i], flags, 5, i, 18, pre=pre)
+            E = eager.problems[i].download()
+            np.testing.assert_array_equal(E[0], U1)
+            np.testing.assert_array_equal(E[1], U2)
+
+
+def test_batch_autotune_keeps_fields():
+    flags = gpu.GR_FAST | gpu.GR_COEF_BF16
+    with gpu.Batch(SIZES[:3], flags=flags, coef=physics.fd8_coef()) as b:
+        _init(b, 21)
+        before = [p.download() for p in b.problems]
+        v, z = b.autotune(2)
+        assert 0 <= v < b.stats()["nvariants"]
+        after = [p.download() for p in b.problems]
+        for x, y in zip(before, after):
+            np.testing.assert_array_equal(x[0], y[0])
+            np.testing.assert_array_equal(x[1], y[1])
+        b.step(3)
+        got = [p.download() for p in b.problems]
+    for i, size in enumerate(SIZES[:3]):
+        U1, U2 = _alone(size, flags, 21, i, 3)
+        np.testing.assert_array_equal(got[i][0], U1)
+        np.testing.assert_array_equal(got[i][1], U2)
+
+
+def test_batch_fp32_bf16_vs_fp64_oracle():
+    """Zero fields, a Ricker source, the layered model: every FP32 problem against
+    the FP64 oracle run on the roc2 values the batch holds (bf16-exact)."""
+    sizes = [(48, 40, 36), (32, 56, 40)]
+    nt = 40
+    dt = 0.4 * 10 / 4725.0
+    src = (0.16 * physics.ricker(30.0, dt, nt + 2, t0=0.012)).astype(np.float32).astype(np.float64)
+    coef = np.zeros(11)
+    coef[:5] = physics.fd8_coef()
+    with gpu.Batch(sizes, flags=gpu.GR_FAST | gpu.GR_COEF_BF16, coef=physics.fd8_coef()) as b:
+        for i, p in enumerate(b.problems):
+            p.init_fields(2)
+            p.init_roc2_model(_model(70 + i))
+            nx, ny, nz = sizes[i]
+            p.set_source(4 + nx // 2, 4 + ny // 2, 4 + nz // 2, src)
+        b.step(nt)
+        outs = [(p.download(), p.download_roc2()) for p in b.problems]
+    for i, ((U1, U2), r2) in enumerate(outs):
+        nx, ny, nz = sizes[i]
+        shp = orc.shape_of(0, nx, ny, nz)
+        u = np.zeros(shp)
+        v = np.zeros(shp)
+        roc2 = np.ascontiguousarray(r2[:, :, :shp[2]].astype(np.float64))
+        pt = (4 + nx // 2, 4 + ny // 2, 4 + nz // 2)
+        orc.run(0, coef, u, v, roc2, nt=nt, src_idx=pt, src=src)
+        for got, ref in ((U1, u), (U2, v)):
+            g = got[:, :, :shp[2]].astype(np.float64)
+            rel = np.linalg.norm(g - ref) / np.linalg.norm(ref)
+            print("problem", i, "rel L2", rel, flush=True)
+            assert np.linalg.norm(ref) > 0
+            assert rel < 1e-5
+
+
+def test_layered_model_layers_and_bf16():
+    nx, ny, nz = 24, 20, 64
+    with gpu.Stepper(0, nx, ny, nz, dtype=gpu.GR_F64, coef=physics.fd8_coef()) as st:
+        st.init_roc2_model(_model(3, nblobs=0, pert=0.0))
+        r2 = st.download_roc2()
+        st.init_roc2_model(_model(3, round_bf16=True))
+        rb = st.download_roc2()
+    inner = r2[4:-4, 4:-4, 4:4 + nx]
+    v = np.sqrt(inner) / DT_OVER_H
+    speeds = np.unique(np.round(v, 6))
+    expect = 1500.0 + 3000.0 * np.arange(8) / 7
+    assert set(np.round(speeds, 3)) <= set(np.round(expect, 3))
+    assert len(speeds) >= 2
+    for k in range(nz):  # layers are horizontal and speeds rise with depth
+        assert np.all(v[k] == v[k, 0, 0])
+        if k:
+            assert v[k, 0, 0] >= v[k - 1, 0, 0]
+    # bf16 values: the low 16 bits of the FP32 pattern are zero
+    bits = rb.astype(np.float32).view(np.uint32)
+    assert np.all(bits & 0xFFFF == 0)
+    assert np.all(rb[4:-4, 4:-4, 4:4 + nx] > 0)
+    with gpu.Stepper(0, nx, ny, nz, dtype=gpu.GR_F32, flags=gpu.GR_COEF_BF16 | gpu.GR_FAST,
+                     coef=physics.fd8_coef()) as st:
+        st.init_roc2_model(_model(3))
+        rbf = st.download_roc2()
+    np.testing.assert_array_equal(rbf[:, :, :nx + 8].astype(np.float64), rb[:, :, :nx + 8])
