@@ -664,25 +664,28 @@ extern "C" int gr_autotune(gr_handle *h, int steps, gr_stats *best) {
       if (!ok) continue;
       const int tl = depths[i];
       const int npass = (steps + tl - 1) / tl;
-      LaunchCfg lc = best_lc;
-      lc.t_block = tl;
-      auto pass = [&]() {
-        if (P.kernel == 0) return tblock_pass(P, tl, ubase(h, 0), ubase(h, 1), a, b, nullptr, -1, lc, h->stream);
-        return tblock_pass(P, tl, nullptr, b, nullptr, a, nullptr, -1, lc, h->stream);
-      };
-      hipError_t e = pass();
-      if (e != hipSuccess) return fail(GR_EHIP, "autotune fused launch: %s", hipGetErrorString(e));
-      HIP_TRY(hipEventRecord(h->ev0, h->stream));
-      for (int q = 0; q < npass; ++q)
-        if ((e = pass()) != hipSuccess) return fail(GR_EHIP, "autotune fused launch: %s", hipGetErrorString(e));
-      HIP_TRY(hipEventRecord(h->ev1, h->stream));
-      HIP_TRY(hipEventSynchronize(h->ev1));
-      float ms = 0.f;
-      HIP_TRY(hipEventElapsedTime(&ms, h->ev0, h->ev1));
-      const float per_step = ms * steps / (float)(npass * tl);  // comparable with the sweep trials
-      if (per_step < best_ms) {
-        best_ms = per_step;
-        best_lc = lc;
+      for (int zc : zchunks) {
+        LaunchCfg lc = best_lc;
+        lc.t_block = tl;
+        lc.zchunk = zc;
+        auto pass = [&]() {
+          if (P.kernel == 0) return tblock_pass(P, tl, ubase(h, 0), ubase(h, 1), a, b, nullptr, -1, lc, h->stream);
+          return tblock_pass(P, tl, nullptr, b, nullptr, a, nullptr, -1, lc, h->stream);
+        };
+        hipError_t e = pass();
+        if (e != hipSuccess) return fail(GR_EHIP, "autotune fused launch: %s", hipGetErrorString(e));
+        HIP_TRY(hipEventRecord(h->ev0, h->stream));
+        for (int q = 0; q < npass; ++q)
+          if ((e = pass()) != hipSuccess) return fail(GR_EHIP, "autotune fused launch: %s", hipGetErrorString(e));
+        HIP_TRY(hipEventRecord(h->ev1, h->stream));
+        HIP_TRY(hipEventSynchronize(h->ev1));
+        float ms = 0.f;
+        HIP_TRY(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+        const float per_step = ms * steps / (float)(npass * tl);  // comparable with the sweep trials
+        if (per_step < best_ms) {
+          best_ms = per_step;
+          best_lc = lc;
+        }
       }
     }
   }

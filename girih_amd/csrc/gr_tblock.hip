@@ -47,6 +47,7 @@ struct TbArgs {
   long long sy, sz;
   int nx, ny, nz;  // interior; boxes are [R, n+R)
   int tiles_x, tiles_y, xcd_order;
+  int zchunk, nzc;  // output planes per workgroup, chunks per tile column
   const T *src;    // source values of this pass (level t adds src[t-1]); src_k < 0: none
   int src_k, src_off;
 };
@@ -64,10 +65,15 @@ __global__ __launch_bounds__(NT) void tb_kernel(const TbArgs<T> a) {
   constexpr int PL = W0 * H0;  // one LDS plane (region-0 coordinates for every level)
   __shared__ T lds[TL][PL];
 
-  const int nb = a.tiles_x * a.tiles_y;
-  const int lb = a.xcd_order ? xcd_remap(blockIdx.x, nb) : (int)blockIdx.x;
-  const int tx0 = R + (lb % a.tiles_x) * TXO;  // tile origin (girih interior indices)
-  const int ty0 = R + (lb / a.tiles_x) * TYO;
+  const int ntile = a.tiles_x * a.tiles_y;
+  const int lb = a.xcd_order ? xcd_remap(blockIdx.x, ntile * a.nzc) : (int)blockIdx.x;
+  const int tile = lb % ntile;
+  const int tx0 = R + (tile % a.tiles_x) * TXO;  // tile origin (girih interior indices)
+  const int ty0 = R + (tile / a.tiles_x) * TYO;
+  // output planes [zs, ze) of this workgroup; level t is needed on
+  // [zs - R*(TL-t), ze + R*(TL-t)), the z analogue of the xy ghost margin
+  const int zs = R + (lb / ntile) * a.zchunk;
+  const int ze = min(zs + a.zchunk, a.nz + R);
   const int nnx = a.nx + 2 * R, nny = a.ny + 2 * R, nnz = a.nz + 2 * R;
   const int tid = threadIdx.x;
 
@@ -136,7 +142,7 @@ __global__ __launch_bounds__(NT) void tb_kernel(const TbArgs<T> a) {
 #pragma unroll
       for (int i = 0; i < Q; ++i) qv[t][j][i] = T(0);
 
-  const int kfirst = -R * TL, klast = a.nz + 2 * R + R * TL;  // k0 range (whole z extent)
+  const int kfirst = zs - R * TL, klast = ze + R * TL;  // k0 range of this chunk
   // prefetch level-0 plane kfirst (only planes inside [0, nnz) exist)
   T n0[NCOL], nh[HS];
   // interior columns of plane k (queue entry) and the halo ring of plane k - R
@@ -179,7 +185,7 @@ __global__ __launch_bounds__(NT) void tb_kernel(const TbArgs<T> a) {
 #pragma unroll
       for (int t = 1; t <= TL; ++t) {
         const int p = k0 - R * t;  // plane computed by level t
-        const bool live = p >= 0 && p < nnz;
+        const bool live = p >= 0 && p < nnz && p >= zs - R * (TL - t);  // earlier planes feed nothing
         const bool zin = p >= R && p < a.nz + R;
         const int sc = ((ph - R * t) % Q + Q) % Q;  // slot of plane p in level t-1's queue
         T val[NCOL];
@@ -253,7 +259,7 @@ __global__ __launch_bounds__(NT) void tb_kernel(const TbArgs<T> a) {
             if (colv[j]) Lt[lofs[j]] = qv[t < TL ? t : 0][j][sl];
         }
         // outputs: level TL into ZN, level TL-1 into ZO (kernel 0) -- planes of the box
-        if (live && zin) {
+        if (live && zin && p >= zs && p < ze) {
           if (t == TL) {
 #pragma unroll
             for (int j = 0; j < NCOL; ++j)
@@ -275,8 +281,10 @@ static hipError_t launch_tb(const TbArgs<T> &a0, hipStream_t st) {
   TbArgs<T> a = a0;
   a.tiles_x = (a.nx + TXO - 1) / TXO;
   a.tiles_y = (a.ny + TYO - 1) / TYO;
-  hipLaunchKernelGGL((tb_kernel<T, KID, NT, TXO, TYO, TL, FAST, ROCS>), dim3(a.tiles_x * a.tiles_y), dim3(NT), 0, st,
-                     a);
+  a.zchunk = (a.zchunk <= 0 || a.zchunk > a.nz) ? a.nz : a.zchunk;
+  a.nzc = (a.nz + a.zchunk - 1) / a.zchunk;
+  hipLaunchKernelGGL((tb_kernel<T, KID, NT, TXO, TYO, TL, FAST, ROCS>), dim3(a.tiles_x * a.tiles_y * a.nzc), dim3(NT),
+                     0, st, a);
   return hipGetLastError();
 }
 
@@ -346,6 +354,7 @@ hipError_t tblock_pass(const Problem &P, int tl, const void *X, const void *Y, v
     a.ny = P.ny;
     a.nz = P.nz;
     a.xcd_order = lc.xcd_order;
+    a.zchunk = lc.zchunk;
     a.src = (const double *)src;
     a.src_k = src_idx < 0 ? -1 : (int)(src_idx / sz);
     a.src_off = src_idx < 0 ? 0 : (int)(src_idx % sz);
@@ -368,6 +377,7 @@ hipError_t tblock_pass(const Problem &P, int tl, const void *X, const void *Y, v
   a.ny = P.ny;
   a.nz = P.nz;
   a.xcd_order = lc.xcd_order;
+  a.zchunk = lc.zchunk;
   a.src = (const float *)src;
   a.src_k = src_idx < 0 ? -1 : (int)(src_idx / sz);
   a.src_off = src_idx < 0 ? 0 : (int)(src_idx % sz);

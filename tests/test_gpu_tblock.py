@@ -43,6 +43,31 @@ def test_fused_bitexact_vs_oracle(kernel, depth, prec):
             np.testing.assert_array_equal(U2[:, :, :nnx], v)
 
 
+@pytest.mark.parametrize("kernel,depth", [(1, 2), (1, 6), (5, 4), (0, 2)])
+@pytest.mark.parametrize("zchunk", [1, 5, 8])
+def test_fused_zchunks_bitexact_vs_oracle(kernel, depth, zchunk):
+    """Workgroups that own a z chunk recompute the chunk's z ghost planes; the
+    result must not depend on the chunk size (including chunks of one plane)."""
+    nx, ny, nz = SIZES[orc.RADIUS[kernel]][0]
+    napply = 2 * depth + (0 if kernel == 0 else 1)
+    print("fused zchunk", kernel, depth, zchunk, flush=True)
+    u, v, roc2, coef = _random_problem(kernel, nx, ny, nz, np.float64, 500 + kernel)
+    with gpu.Stepper(kernel, nx, ny, nz, dtype=gpu.GR_F64, coef=_consts(coef, kernel)) as st:
+        st.upload(_pad(u), _pad(v), _pad(roc2) if roc2 is not None else None, _pad_coef(coef, kernel))
+        st.set_config(zchunk=zchunk, t_block=depth)
+        st.step(napply)
+        assert st.stats()["last_launches"] < napply
+        U1, U2 = st.download(_pad(u).shape)
+    for a in range(napply):
+        if a % 2 == 0:
+            orc.sweep(kernel, coef, u, v, roc2)
+        else:
+            orc.sweep(kernel, coef, v, u, roc2)
+    nnx = u.shape[2]
+    np.testing.assert_array_equal(U1[:, :, :nnx], u)
+    np.testing.assert_array_equal(U2[:, :, :nnx], v)
+
+
 def test_fused_wave_with_source_fp64():
     nx, ny, nz = 66, 40, 30
     kernel = 0
