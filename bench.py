@@ -3,14 +3,21 @@
 Workload: u^(n+1) = 2u^n - u^(n-1) + (v dt/h)^2 Lap8(u^n), 8th-order FD weights,
 v = 3000 m/s, h = 10 m, dt at CFL 0.4 (roc2 = 0.16), 4-cell halo, 15 Hz Ricker
 point source at the grid centre, zero initial fields; one "step" = one time step
-over the whole 1024^3 grid.  The timed run is the FP32 path (FP32 fields, neighbour
-pairs summed before scaling, compensated 2u^n - u^(n-1) update); the same steps are
-then run in FP64 (the GPU oracle) and the FP32 error is reduced on the device in FP64.
+over the whole 1024^3 grid.  The timed run is the FP32 path (FP32 fields,
+neighbour pairs summed before scaling, increment-form leapfrog update); the same
+steps are then run in FP64 (bit-identical to girih's serial reference) and the
+FP32 error is reduced on the device in FP64.
 
-Prints ONE JSON line (rank 0, N = 1).  --gpus N > 1 is refused: this build runs on
-one GPU.
+Prints ONE JSON line (N = 1).  --gpus N > 1 is refused: this build runs on one GPU.
 
     python bench.py [--steps 1000] [--warmup 10] [--n 1024] [--no-fp64] [--no-cpu-baseline]
+                    [--no-extras] [--config VARIANT,ZCHUNK,XCD_ORDER,T_BLOCK]
+
+--config pins the launch configuration instead of autotuning it (profiler runs
+use the configuration the tuned run printed).  The "extras" object measures the
+other workloads of the same stepper: the 7-point Jacobi at 128^3 (FP64), the
+7-point variable-coefficient stencil at 512^3 (FP64 and FP32, with the FP32
+error), and the batch of 16 uneven 25-point problems with bf16 roc2.
 """
 import argparse
 import json
@@ -31,11 +38,21 @@ def _sync():
     torch.cuda.synchronize()
 
 
-def run_wave(dtype, flags, cfg, steps, warmup):
+def _configure(st, config, tune_steps=4):
+    """Autotune (tile variant x z chunk x tile order x fused depth) or pin a config."""
+    if config is None:
+        return st.autotune(tune_steps)
+    var, zc, order, tb = config
+    st.set_config(var, zc, tb)
+    st.set_xcd_order(order)
+    return st.stats()
+
+
+def run_wave(dtype, flags, cfg, steps, warmup, config=None):
     from girih_amd import gpu
     st = gpu.Stepper(0, cfg["nx"], cfg["ny"], cfg["nz"], dtype=dtype, flags=flags | gpu.GR_ROC2_SCALAR,
                      roc2=cfg["roc2"], coef=cfg["coef"])
-    tuned = st.autotune(4)  # tile shape x z-chunk search on scratch copies of the fields
+    tuned = _configure(st, config)
     if warmup > 0:
         st.init_fields(2)
         st.step(warmup)
@@ -65,7 +82,10 @@ def cpu_baseline(n, budget_nt):
     env = dict(os.environ, OMP_NUM_THREADS=str(cores), OMP_PROC_BIND="true", OMP_PLACES="cores",
                OMP_WAIT_POLICY="active")
     t0 = time.perf_counter()
-    r = subprocess.run(cmd, env=env, capture_output=True, text=True, timeout=600)
+    try:
+        r = subprocess.run(cmd, env=env, capture_output=True, text=True, timeout=600)
+    except (OSError, subprocess.SubprocessError) as e:
+        return {"value": None, "unit": "GLUP/s", "cores": cores, "kind": "reference", "sample": "failed: %s" % e}
     wall = time.perf_counter() - t0
     m = re.search(r"Total RANK0 MStencil/s MAX:\s*([0-9.eE+-]+)", r.stdout)
     if r.returncode != 0 or not m:
@@ -78,7 +98,6 @@ def cpu_baseline(n, budget_nt):
 
 def cpu_baseline_port():
     sys.path.insert(0, os.path.join(REPO, "oracle"))
-    import numpy as np
     import oracle as orc
     n, nt = 128, 4
     u, v, roc2 = orc.init_ref(0, n, n, n)
@@ -98,6 +117,98 @@ def load_traffic(key):
         return json.load(f).get(key)
 
 
+def _timed_steps(st, napply):
+    _sync()
+    t0 = time.perf_counter()
+    st.step(napply)
+    _sync()
+    return time.perf_counter() - t0
+
+
+def extra_first_order(kernel, n, nt, coef=None, fp32=True):
+    """girih kernels 1 / 5: FP64 (bit-exact path) and optionally FP32 on the same
+    seeded input, autotuned; FP32 error vs FP64 reduced on the device."""
+    from girih_amd import gpu
+    napply = 2 * ((nt + 1) // 2)  # girih's SB/MWD application count for even nt
+    out = {}
+    runs = [("f64", gpu.GR_F64)] + ([("f32", gpu.GR_F32)] if fp32 else [])
+    keep = {}
+    for name, dtype in runs:
+        st = gpu.Stepper(kernel, n, n, n, dtype=dtype, coef=coef)
+        if gpu.NUM_COEF_ARRAYS[kernel]:
+            st.init_coef_uniform(5, 0.3, 0.5, 0.0, 0.1)
+        st.init_fields(1, 12345)
+        tuned = st.autotune(4)
+        st.init_fields(1, 12345)
+        t = _timed_steps(st, napply)
+        s = st.stats()
+        b_sweep = (2 + gpu.NUM_COEF_ARRAYS[kernel]) * (8 if dtype == gpu.GR_F64 else 4)
+        gl = n ** 3 * napply / t / 1e9
+        out[name] = {"glups": round(gl, 2), "ms_per_step": round(t * 1e3 / napply, 4),
+                     "sweep_roofline_frac": round(gl * b_sweep / HBM_PEAK_GBS, 4), "bytes_per_lup_sweep": b_sweep,
+                     "t_block": tuned["t_block"], "zchunk": tuned["zchunk"], "tile": [tuned["tile_x"],
+                                                                                     tuned["tile_y"]],
+                     "launches": s["last_launches"]}
+        keep[name] = st
+    if "f32" in keep:
+        c = keep["f32"].compare(1, keep["f64"], 1)
+        out["f32"]["rel_l2_vs_f64"] = c["rel_l2"]
+    for st in keep.values():
+        st.close()
+    return out
+
+
+def extra_batch(nt, seed=2024):
+    """16 independent 25-pt problems, dims uniform in 192..512 (multiples of 16),
+    8-layer velocity model with +-5% Gaussian perturbation, bf16 roc2, one
+    persistent work-queue launch per step replayed from a HIP graph; each problem
+    checked against the FP64 path (bit-identical to the CPU oracle) on the same
+    bf16 roc2 values."""
+    import numpy as np
+    from girih_amd import gpu, physics
+    rng = np.random.default_rng(seed)
+    sizes = [tuple(int(x) for x in rng.choice(np.arange(192, 513, 16), 3)) for _ in range(16)]
+    vmax = 4500.0 * 1.05
+    h = 10.0
+    dt = physics.cfl_dt(vmax, h, 0.4)
+    f0 = 15.0
+    src = physics.roc2_const(3000.0, dt, h) * physics.ricker(f0, dt, nt + 2)
+    cells = sum(a * b * c for a, b, c in sizes)
+    with gpu.Batch(sizes, flags=gpu.GR_FAST | gpu.GR_COEF_BF16, coef=physics.fd8_coef()) as b:
+        for i, p in enumerate(b.problems):
+            p.init_fields(2)
+            p.init_roc2_model(gpu.vmodel(seed + i, dt / h))
+            nx, ny, nz = sizes[i]
+            p.set_source(4 + nx // 2, 4 + ny // 2, 4 + nz // 2, src)
+        var, zc = b.autotune(4, use_graph=True)
+        b.step(nt, use_graph=True)  # captures the step-loop graph, then runs it
+        t_first = b.stats()["last_ms"]
+        for i, p in enumerate(b.problems):  # restart from the initial state, replay the captured graph
+            p.init_fields(2)
+            nx, ny, nz = sizes[i]
+            p.set_source(4 + nx // 2, 4 + ny // 2, 4 + nz // 2, src)
+        b.step(nt, use_graph=True)
+        s = b.stats()
+        errs = []
+        for i, p in enumerate(b.problems):
+            nx, ny, nz = sizes[i]
+            with gpu.Stepper(0, nx, ny, nz, dtype=gpu.GR_F64, coef=physics.fd8_coef()) as ref:
+                ref.init_fields(2)
+                ref.init_roc2_model(gpu.vmodel(seed + i, dt / h, round_bf16=True))
+                ref.set_source(4 + nx // 2, 4 + ny // 2, 4 + nz // 2, src)
+                ref.set_config(zchunk=64)
+                ref.step(nt)
+                f = (nt - 1) % 2
+                errs.append(p.compare(f, ref, f)["rel_l2"])
+    gl = cells * nt / (s["last_ms"] * 1e-3) / 1e9
+    return {"glups": round(gl, 2), "ms_per_step": round(s["last_ms"] / nt, 4), "problems": len(sizes),
+            "cells": cells, "steps": nt, "sizes": sizes, "items_per_step": s["items"],
+            "persistent_workgroups": s["grid"], "tile": [s["tile_x"], s["tile_y"]], "zchunk": s["zchunk"],
+            "graph_first_call_ms": round(t_first, 2),
+            "sweep_roofline_frac": round(gl * 14 / HBM_PEAK_GBS, 4), "bytes_per_lup_sweep": 14,
+            "rel_l2_vs_f64_max": max(errs), "rel_l2_vs_f64": [float("%.3g" % e) for e in errs]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -106,11 +217,17 @@ def main():
     ap.add_argument("--n", type=int, default=1024)
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-nt", type=int, default=10)
+    ap.add_argument("--no-extras", action="store_true")
+    ap.add_argument("--cpu-nt", type=int, default=60)
+    ap.add_argument("--config", default=None, help="VARIANT,ZCHUNK,XCD_ORDER,T_BLOCK (skips the autotuner)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on one GPU only; --gpus %d is not supported "
               "(the z-slab sharding across GPUs is designed, not built)" % args.gpus, file=sys.stderr)
+        return 2
+    config = tuple(int(x) for x in args.config.split(",")) if args.config else None
+    if config is not None and len(config) != 4:
+        print("bench.py: --config needs VARIANT,ZCHUNK,XCD_ORDER,T_BLOCK", file=sys.stderr)
         return 2
 
     import torch
@@ -120,16 +237,17 @@ def main():
     cfg = physics.wave_1024_config(nx=n, ny=n, nz=n, nt=K)
     cells = n * n * n
 
-    st32, t32, s32 = run_wave(gpu.GR_F32, gpu.GR_FAST, cfg, K, W)
+    st32, t32, s32 = run_wave(gpu.GR_F32, gpu.GR_FAST, cfg, K, W, config)
     glups32 = cells * K / t32 / 1e9
     launch_ms = s32["last_ms"] / max(1, s32["last_launches"])
     steps_per_launch = K / max(1, s32["last_launches"])
     # algorithmic HBM bytes per launch: read u^n, u^(n-1), write u^(n+1) per cell and
-    # fused step (4 B each, roc2 is a scalar); a fused launch streams 2 fields in, 2 out
+    # step (4 B each, roc2 is a scalar); a fused launch streams 2 fields in, 2 out
     bytes_per_cell_launch = 12.0 if steps_per_launch == 1 else 16.0
     algo_bytes = bytes_per_cell_launch * cells
     achieved = algo_bytes / (launch_ms * 1e-3) / 1e9
-    traffic = load_traffic("wave25_f32_%d" % n)
+    tr = load_traffic("wave25_f32_%d" % n)
+    traffic = tr["bytes_per_launch"] if tr and abs(tr.get("steps_per_launch", 1) - steps_per_launch) < 1e-9 else None
 
     out = {
         "metric": "GLUP/s + % HBM roofline, 25pt wave 1024^3; FP32 vs FP64 GPU at stated error",
@@ -145,26 +263,40 @@ def main():
         "dtype": "f32",
         "data": "synthetic (zero initial field, Ricker point source)",
         "config": {"workload": "25-pt star wave (girih kernel 0), %d^3 + 4-cell halo, %d steps, v=3000 m/s, "
-                               "h=10 m, CFL 0.4, 15 Hz Ricker at centre, FP32 fields, compensated update" % (n, K),
+                               "h=10 m, CFL 0.4, 15 Hz Ricker at centre, FP32 fields, increment-form update"
+                               % (n, K),
                    "nx": n, "ny": n, "nz": n, "nt": K, "t_block": s32["t_block"],
-                   "tile": [s32["tile_x"], s32["tile_y"]], "zchunk": s32["zchunk"]},
+                   "tile": [s32["tile_x"], s32["tile_y"]], "zchunk": s32["zchunk"],
+                   "launch_config": "%d,%d,%d,%d" % (s32["tile_variant"], s32["zchunk"], s32["xcd_order"],
+                                                     s32["t_block"]),
+                   "autotuned": config is None},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "algo_bytes_per_launch": algo_bytes, "avg_launch_ms": round(launch_ms, 5),
-                     "steps_per_launch": steps_per_launch},
+                     "steps_per_launch": steps_per_launch,
+                     "traffic_bytes_per_lup": (round(traffic / (cells * steps_per_launch), 3) if traffic else None)},
     }
     if not args.no_fp64:
         st64, t64, s64 = run_wave(gpu.GR_F64, 0, cfg, K, W)
         glups64 = cells * K / t64 / 1e9
-        field = (K - 1) % 2  # after K applications the last written field is U1 (odd) / U2 (even)
+        field = (K - 1) % 2  # after K applications the last written field is U1 (odd K) / U2 (even K)
         cmp = st32.compare(field, st64, field)
         out["fp64"] = {"glups": round(glups64, 3), "ms_per_step": round(t64 * 1e3 / K, 4),
                        "fp32_over_fp64": round(glups32 / glups64, 3),
                        "rel_l2_fp32_vs_fp64": cmp["rel_l2"], "max_abs_err": cmp["max_diff"],
-                       "max_abs_ref": cmp["max_ref"]}
+                       "max_abs_ref": cmp["max_ref"], "roofline_frac": round(glups64 * 24 / HBM_PEAK_GBS, 4),
+                       "t_block": s64["t_block"], "tile": [s64["tile_x"], s64["tile_y"]], "zchunk": s64["zchunk"]}
         st64.close()
     st32.close()
+    if not args.no_extras:
+        print("bench: extras", file=sys.stderr, flush=True)
+        out["extras"] = {
+            "jacobi7_f64_128": extra_first_order(1, 128, 100, coef=[0.4, 0.1], fp32=False),
+            "var7_nosym_512": extra_first_order(5, 512, 500),
+            "batch16_wave25_bf16": extra_batch(500),
+        }
     if not args.no_cpu_baseline:
+        print("bench: cpu baseline", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(n, args.cpu_nt)
     print(json.dumps(out))
     return 0

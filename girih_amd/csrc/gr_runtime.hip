@@ -557,6 +557,8 @@ extern "C" int gr_get_stats(gr_handle *h, gr_stats *out) {
   s.rows_y = h->P.rows;
   s.planes_z = h->P.planes;
   s.x_offset = h->P.xoff;
+  s.tile_variant = h->lc.variant;
+  s.xcd_order = h->lc.xcd_order;
   *out = s;
   return GR_OK;
 }
@@ -580,6 +582,13 @@ extern "C" int gr_set_config(gr_handle *h, int variant, int zchunk, int t_block)
     }
     h->lc.t_block = t_block;
   }
+  return GR_OK;
+}
+
+extern "C" int gr_set_xcd_order(gr_handle *h, int xcd_order) {
+  if (!h) return fail(GR_EINVAL, "gr_set_xcd_order: null handle");
+  if (xcd_order != 0 && xcd_order != 1) return fail(GR_EINVAL, "gr_set_xcd_order: 0 or 1");
+  h->lc.xcd_order = xcd_order;
   return GR_OK;
 }
 

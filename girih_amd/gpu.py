@@ -47,7 +47,7 @@ class Stats(ctypes.Structure):
                 ("t_block", ctypes.c_int32), ("tile_x", ctypes.c_int32), ("tile_y", ctypes.c_int32),
                 ("zchunk", ctypes.c_int32), ("parity", ctypes.c_int32), ("device_bytes", ctypes.c_int64),
                 ("pitch_x", ctypes.c_int32), ("rows_y", ctypes.c_int32), ("planes_z", ctypes.c_int32),
-                ("x_offset", ctypes.c_int32)]
+                ("x_offset", ctypes.c_int32), ("tile_variant", ctypes.c_int32), ("xcd_order", ctypes.c_int32)]
 
 
 class VModel(ctypes.Structure):
@@ -67,7 +67,8 @@ class BatchStats(ctypes.Structure):
 
 EXPORTS = ["gr_create", "gr_destroy", "gr_last_error", "gr_upload", "gr_download", "gr_step", "gr_run_ts",
            "gr_set_source", "gr_get_stats", "gr_mwd_round_nt", "gr_sweep_dev", "gr_field_ptr", "gr_init_fields",
-           "gr_init_coef_uniform", "gr_compare_fields", "gr_set_config", "gr_autotune", "gr_init_roc2_model",
+           "gr_init_coef_uniform", "gr_compare_fields", "gr_set_config", "gr_set_xcd_order", "gr_autotune",
+           "gr_init_roc2_model",
            "gr_download_roc2", "gr_batch_create", "gr_batch_destroy", "gr_batch_problem", "gr_batch_set_config",
            "gr_batch_step", "gr_batch_autotune", "gr_batch_get_stats"]
 
@@ -107,6 +108,7 @@ def lib():
                                       ctypes.c_double], I),
             "gr_compare_fields": ([P, I, P, I, ctypes.POINTER(ctypes.c_double)], I),
             "gr_set_config": ([P, I, I, I], I),
+            "gr_set_xcd_order": ([P, I], I),
             "gr_autotune": ([P, I, ctypes.POINTER(Stats)], I),
             "gr_init_roc2_model": ([P, ctypes.POINTER(VModel)], I),
             "gr_download_roc2": ([P, IP, P], I),
@@ -269,6 +271,9 @@ class Stepper:
 
     def set_config(self, tile_variant=-1, zchunk=-1, t_block=-1):
         check(lib().gr_set_config(self._h, int(tile_variant), int(zchunk), int(t_block)))
+
+    def set_xcd_order(self, order):
+        check(lib().gr_set_xcd_order(self._h, int(order)))
 
     def autotune(self, steps=4):
         s = Stats()

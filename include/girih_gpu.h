@@ -77,6 +77,8 @@ typedef struct gr_stats {
   int64_t device_bytes;      /* HBM held by the handle */
   int32_t pitch_x, rows_y, planes_z; /* device layout of one field */
   int32_t x_offset;          /* element offset of logical x = 0 within a row */
+  int32_t tile_variant;      /* gr_set_config tile variant in use */
+  int32_t xcd_order;         /* 1: workgroups renumbered so each XCD owns a contiguous tile range */
 } gr_stats;
 
 /* ---- Level 1: time stepper (replaces TSList[ts].func, src/wrappers.h:28-36) ---- */
@@ -149,6 +151,8 @@ int gr_init_coef_uniform(gr_handle *h, uint64_t seed, double lo0, double hi0, do
  * fused time steps per HBM pass (1 = single sweep).  Values < 0 leave a setting
  * unchanged.  Results never depend on the configuration. */
 int gr_set_config(gr_handle *h, int tile_variant, int zchunk, int t_block);
+/* Workgroup order of the sweep kernels: 1 = XCD-contiguous tile ranges, 0 = plain. */
+int gr_set_xcd_order(gr_handle *h, int xcd_order);
 /* Autotuner (replaces girih's CPU tile/thread-group search, diamond_utils.c:691-847):
  * times every tile variant x z-chunk x fused depth candidate for `steps` steps on
  * scratch copies of the fields, keeps the fastest, and reports it in *best (may be
