@@ -1,0 +1,94 @@
+"""Seismic shots on the 25-point wave stepper, differentiable in torch.
+
+    shot = Shot(nx, ny, nz, nt, dt_over_h, src_point, src, rec_k, ckpt=50)
+    traces = shot_traces(m, shot)                  # torch.autograd.Function
+    loss = 0.5 * ((traces - d_obs) ** 2).sum()
+    loss.backward()                                # m.grad = dJ/dm by the adjoint
+
+m is the squared slowness 1/v^2 per interior cell, a dense CUDA tensor
+[nz, ny, nx] of the field type; traces are [nt, ny, nx] (the interior of plane
+rec_k after every application).  The forward and the vector-Jacobian product run
+in libgirih_gpu.so (gr_shot_*): the same HIP sweep kernels as the stepper,
+checkpoints every `ckpt` applications, recomputation in the backward pass.  There
+is no CPU path: without the library every call raises.
+"""
+import ctypes
+
+import torch
+
+from . import gpu, physics
+
+
+class Shot:
+    """A kernel-0 problem with a point source and a receiver plane (gr_shot)."""
+
+    def __init__(self, nx, ny, nz, nt, dt_over_h, src_point, src, rec_k, ckpt=50, dtype=gpu.GR_F32, coef=None):
+        flags = gpu.GR_FAST if dtype == gpu.GR_F32 else 0
+        self.nx, self.ny, self.nz, self.nt = nx, ny, nz, nt
+        self.dt_over_h = float(dt_over_h)
+        self.dtype = dtype
+        self.torch_dtype = torch.float32 if dtype == gpu.GR_F32 else torch.float64
+        self.stepper = gpu.Stepper(0, nx, ny, nz, dtype=dtype, flags=flags,
+                                   coef=physics.fd8_coef() if coef is None else coef)
+        self.stepper.set_config(zchunk=64)
+        self.stepper.set_source(*src_point, src)
+        self._s = ctypes.c_void_p()
+        gpu.check(gpu.lib().gr_shot_create(self.stepper._h, int(rec_k), int(nt), int(ckpt), self.dt_over_h,
+                                           ctypes.byref(self._s)))
+        self._m_version = None  # the m of the last forward run (checkpoints belong to it)
+
+    def close(self):
+        if self._s:
+            gpu.lib().gr_shot_destroy(self._s)
+            self._s = ctypes.c_void_p()
+        self.stepper.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, t, shape):
+        if not (t.is_cuda and t.dtype == self.torch_dtype and t.is_contiguous() and tuple(t.shape) == shape):
+            raise ValueError("expected a contiguous CUDA %s tensor of shape %s" % (self.torch_dtype, shape))
+
+    def forward(self, m):
+        """Traces [nt, ny, nx] for squared slowness m [nz, ny, nx]."""
+        self._check(m, (self.nz, self.ny, self.nx))
+        traces = torch.empty((self.nt, self.ny, self.nx), dtype=self.torch_dtype, device=m.device)
+        torch.cuda.current_stream(m.device).synchronize()  # m is written by torch's stream
+        gpu.check(gpu.lib().gr_set_roc2_from_slowness2(self.stepper._h, ctypes.c_void_p(m.data_ptr()),
+                                                       self.dt_over_h))
+        gpu.check(gpu.lib().gr_shot_forward(self._s, ctypes.c_void_p(traces.data_ptr())))
+        self._m_version = (m.data_ptr(), m._version)
+        return traces
+
+    def vjp(self, m, res):
+        """sum_a <res[a], d traces[a] / dm>: the adjoint gradient [nz, ny, nx]."""
+        res = res.contiguous()
+        self._check(res, (self.nt, self.ny, self.nx))
+        if self._m_version != (m.data_ptr(), m._version):  # checkpoints are of another m: recompute them
+            self.forward(m)
+        grad = torch.empty((self.nz, self.ny, self.nx), dtype=self.torch_dtype, device=res.device)
+        torch.cuda.current_stream(res.device).synchronize()
+        gpu.check(gpu.lib().gr_shot_adjoint(self._s, ctypes.c_void_p(res.data_ptr()), ctypes.c_void_p(grad.data_ptr())))
+        return grad
+
+
+class _ShotTraces(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, m, shot):
+        ctx.shot = shot
+        ctx.save_for_backward(m)
+        return shot.forward(m.detach())
+
+    @staticmethod
+    def backward(ctx, res):
+        (m,) = ctx.saved_tensors
+        return ctx.shot.vjp(m.detach(), res), None
+
+
+def shot_traces(m, shot):
+    """Differentiable receiver traces of `shot` for squared slowness m."""
+    return _ShotTraces.apply(m, shot)

@@ -21,7 +21,7 @@ DRIVER_DIR = os.path.join(PKG, "driver")
 DRIVER_BINS = {"dp": os.path.join(DRIVER_DIR, "bin", "mwd_kernel_gpu_dp"),
                "sp": os.path.join(DRIVER_DIR, "bin", "mwd_kernel_gpu_sp")}
 
-HIP_SOURCES = ["gr_runtime.hip", "gr_tblock.hip", "gr_model.hip", "gr_batch.hip"]
+HIP_SOURCES = ["gr_runtime.hip", "gr_tblock.hip", "gr_model.hip", "gr_batch.hip", "gr_adjoint.hip"]
 ARCH = os.environ.get("GIRIH_OFFLOAD_ARCH", "gfx950")
 
 # -ffp-contract=off: the reference-order kernels must not fuse multiply-adds

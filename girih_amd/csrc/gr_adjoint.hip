@@ -1,0 +1,345 @@
+// gr_adjoint.hip -- seismic shots on the kernel-0 wave stepper: receiver traces,
+// and the gradient of any function of the traces with respect to the squared
+// slowness m = 1/v^2, by the discrete adjoint with checkpointing.
+//
+// Forward (girih's kernel 0 with a per-cell roc2 = (v dt/h)^2 = dt^2/(h^2 m),
+// src/kernels/stencils.c:27-41, source as nb_naive_ts.c:190,196):
+//   x_0 = x_1 = 0,  x_{a+2} = 2 x_{a+1} - x_a + r (L x_{a+1}) + s_a e_src,
+//   trace a = P x_{a+2} (the interior of plane rec_k),       a = 0 .. nt-1.
+// L is the symmetric star stencil on the interior (girih's halo is a zero,
+// never-updated boundary).  For a cotangent res_a of trace a, the adjoint in the
+// scaled variable nu = r * mu (mu the Lagrange multiplier of x) is the SAME
+// leapfrog, run backwards with the receivers as sources:
+//   nu_{a+2} = 2 nu_{a+3} - nu_{a+4} + r (L nu_{a+3}) + r P^T res_a,  nu_{nt+2} = nu_{nt+3} = 0,
+// so the forward sweep kernel steps it unchanged.  With r L x_{a+1} =
+// x_{a+2} - 2x_{a+1} + x_a - s_a e_src and dr/dm = -r^2 h^2/dt^2:
+//   dJ/dm = -(h/dt)^2 * sum_a nu_{a+2} * ((x_{a+2} - x_{a+1}) - (x_{a+1} - x_a) - s_a e_src),
+// a pointwise accumulation (FP64) with no second stencil pass.
+//
+// The forward pass stores (x_{cK}, x_{cK+1}) every K applications; the backward
+// pass recomputes each segment's K states from its checkpoint into a buffer and
+// walks it backwards.  Memory: 2*ceil(nt/K) + K + 4 fields + an FP64 accumulator.
+#include <hip/hip_runtime.h>
+#include <hip/hip_bf16.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "../../include/girih_gpu.h"
+#include "gr_internal.hpp"
+#include "gr_sweep.hpp"
+
+namespace {
+
+using gr::Problem;
+
+template <typename T>
+__global__ void record_kernel(const T *x, long long sy, long long plane_off, int nx, int ny, int R, T *rec) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+  if (i < nx) rec[(long long)j * nx + i] = x[plane_off + (long long)(j + R) * sy + (i + R)];
+}
+
+// nu(plane rec_k) += roc2 * res  (the adjoint source of the traces)
+template <typename T, typename C>
+__global__ void inject_kernel(T *nu, const C *roc2, long long sy, long long plane_off, int nx, int ny, int R,
+                              const T *res) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+  if (i >= nx) return;
+  const long long o = plane_off + (long long)(j + R) * sy + (i + R);
+  nu[o] = nu[o] + (T)(float)roc2[o] * res[(long long)j * nx + i];
+}
+template <typename T>
+__global__ void inject_kernel_same(T *nu, const T *roc2, long long sy, long long plane_off, int nx, int ny, int R,
+                                   const T *res) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+  if (i >= nx) return;
+  const long long o = plane_off + (long long)(j + R) * sy + (i + R);
+  nu[o] = nu[o] + roc2[o] * res[(long long)j * nx + i];
+}
+
+// g += nu * ((x2 - x1) - (x1 - x0) - s e_src), in FP64 over the interior (g dense [nz][ny][nx])
+template <typename T>
+__global__ void grad_accum_kernel(double *g, const T *nu, const T *x2, const T *x1, const T *x0, long long sy,
+                                  long long sz, int nx, int ny, int R, const T *src, long long src_idx) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y, k = blockIdx.z;
+  if (i >= nx) return;
+  const long long o = (long long)(k + R) * sz + (long long)(j + R) * sy + (i + R);
+  const double a = (double)x2[o], b = (double)x1[o], c = (double)x0[o];
+  double d = (a - b) - (b - c);
+  if (src && o == src_idx) d = d - (double)*src;
+  g[((long long)k * ny + j) * nx + i] += (double)nu[o] * d;
+}
+
+template <typename T>
+__global__ void grad_final_kernel(const double *g, T *out, long long n, double scale) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) out[t] = (T)(scale * g[t]);
+}
+
+__global__ void zero_kernel(double *g, long long n) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) g[t] = 0.0;
+}
+
+// roc2 = dt_over_h^2 / m over the whole field (halo cells take the nearest interior m)
+template <typename T, typename C>
+__global__ void roc2_from_m_kernel(C *roc2, const T *m, int nx, int ny, int nz, int R, long long pitch,
+                                   long long rows, long long total, double k2) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= total) return;
+  const int nnx = nx + 2 * R;
+  const long long row = t / nnx;
+  const int i = (int)(t - row * nnx);
+  const int j = (int)(row % rows), k = (int)(row / rows);
+  const int ii = min(max(i - R, 0), nx - 1), jj = min(max(j - R, 0), ny - 1), kk = min(max(k - R, 0), nz - 1);
+  const double mv = (double)m[((long long)kk * ny + jj) * nx + ii];
+  const double r = k2 / mv;
+  if constexpr (std::is_same<C, __hip_bfloat16>::value)
+    roc2[row * pitch + i] = __float2bfloat16((float)r);
+  else
+    roc2[row * pitch + i] = (C)r;
+}
+
+}  // namespace
+
+struct gr_shot {
+  gr_handle *h = nullptr;
+  int rec_k = 0, nt = 0, ckpt = 0, nck = 0;
+  double dt_over_h = 0;
+  std::vector<void *> allocs;  // everything below lives in these
+  std::vector<void *> ck;      // 2*nck checkpoint fields (x_{cK}, x_{cK+1})
+  std::vector<void *> seg;     // ckpt+2 recompute fields
+  void *nu[2] = {nullptr, nullptr};
+  void *x[2] = {nullptr, nullptr};  // forward ping-pong
+  double *g = nullptr;
+  bool have_forward = false;
+};
+
+namespace {
+
+int shot_fail(hipError_t e, const char *what) { return gr::set_errorf(GR_EHIP, "%s: %s", what, hipGetErrorString(e)); }
+
+// field base (x offset applied) of a raw field allocation
+void *fbase(const Problem &P, void *alloc) { return (char *)alloc + (size_t)P.xoff * gr::elem_size(P.dtype); }
+
+template <typename T>
+hipError_t record(const Problem &P, const void *x, int rec_k, void *rec, hipStream_t st) {
+  const long long sz = (long long)P.pitch * P.rows;
+  hipLaunchKernelGGL(record_kernel<T>, dim3((P.nx + 255) / 256, P.ny), dim3(256), 0, st, (const T *)x,
+                     (long long)P.pitch, (long long)rec_k * sz, P.nx, P.ny, P.R, (T *)rec);
+  return hipGetLastError();
+}
+
+hipError_t record_any(const Problem &P, const void *x, int rec_k, void *rec, hipStream_t st) {
+  return P.dtype == GR_F64 ? record<double>(P, x, rec_k, rec, st) : record<float>(P, x, rec_k, rec, st);
+}
+
+hipError_t inject_any(const Problem &P, void *nu, int rec_k, const void *res, hipStream_t st) {
+  const long long sz = (long long)P.pitch * P.rows;
+  const dim3 grid((P.nx + 255) / 256, P.ny);
+  if (P.dtype == GR_F64)
+    hipLaunchKernelGGL(inject_kernel_same<double>, grid, dim3(256), 0, st, (double *)nu, (const double *)P.roc2,
+                       (long long)P.pitch, (long long)rec_k * sz, P.nx, P.ny, P.R, (const double *)res);
+  else if (P.flags & GR_COEF_BF16)
+    hipLaunchKernelGGL((inject_kernel<float, __hip_bfloat16>), grid, dim3(256), 0, st, (float *)nu,
+                       (const __hip_bfloat16 *)P.roc2, (long long)P.pitch, (long long)rec_k * sz, P.nx, P.ny, P.R,
+                       (const float *)res);
+  else
+    hipLaunchKernelGGL(inject_kernel_same<float>, grid, dim3(256), 0, st, (float *)nu, (const float *)P.roc2,
+                       (long long)P.pitch, (long long)rec_k * sz, P.nx, P.ny, P.R, (const float *)res);
+  return hipGetLastError();
+}
+
+hipError_t grad_any(const Problem &P, double *g, const void *nu, const void *x2, const void *x1, const void *x0,
+                    const void *src, long long sidx, hipStream_t st) {
+  const long long sz = (long long)P.pitch * P.rows;
+  const dim3 grid((P.nx + 255) / 256, P.ny, P.nz);
+  if (P.dtype == GR_F64)
+    hipLaunchKernelGGL(grad_accum_kernel<double>, grid, dim3(256), 0, st, g, (const double *)nu, (const double *)x2,
+                       (const double *)x1, (const double *)x0, (long long)P.pitch, sz, P.nx, P.ny, P.R,
+                       (const double *)src, sidx);
+  else
+    hipLaunchKernelGGL(grad_accum_kernel<float>, grid, dim3(256), 0, st, g, (const float *)nu, (const float *)x2,
+                       (const float *)x1, (const float *)x0, (long long)P.pitch, sz, P.nx, P.ny, P.R,
+                       (const float *)src, sidx);
+  return hipGetLastError();
+}
+
+// source value pointer of application a (null when none)
+const void *src_of(gr_handle *h, long long a, long long *sidx) {
+  int n = 0;
+  const void *s = gr_internal_source(h, &n, sidx);
+  if (!s || *sidx < 0 || a >= n) {
+    *sidx = -1;
+    return nullptr;
+  }
+  return (const char *)s + (size_t)a * gr::elem_size(gr_internal_problem(h).dtype);
+}
+
+}  // namespace
+
+extern "C" int gr_shot_destroy(gr_shot *s) {
+  if (!s) return GR_OK;
+  if (s->h) (void)hipStreamSynchronize(gr_internal_stream(s->h));
+  for (void *p : s->allocs)
+    if (p) (void)hipFree(p);
+  delete s;
+  return GR_OK;
+}
+
+extern "C" int gr_shot_create(gr_handle *h, int rec_k, int nt, int ckpt, double dt_over_h, gr_shot **out) {
+  if (!h || !out) return gr::set_error(GR_EINVAL, "gr_shot_create: null argument");
+  *out = nullptr;
+  const Problem &P = gr_internal_problem(h);
+  if (P.kernel != 0 || !P.roc2)
+    return gr::set_error(GR_EINVAL, "gr_shot_create: shots run kernel 0 with a roc2 field (not GR_ROC2_SCALAR)");
+  if (rec_k < P.R || rec_k >= P.nz + P.R)
+    return gr::set_errorf(GR_EINVAL, "gr_shot_create: receiver plane %d is not an interior plane", rec_k);
+  if (nt < 1 || ckpt < 1 || !(dt_over_h > 0))
+    return gr::set_error(GR_EINVAL, "gr_shot_create: need nt >= 1, ckpt >= 1, dt_over_h > 0");
+  gr_shot *s = new gr_shot();
+  s->h = h;
+  s->rec_k = rec_k;
+  s->nt = nt;
+  s->ckpt = std::min(ckpt, nt);
+  s->nck = (nt + s->ckpt - 1) / s->ckpt;
+  s->dt_over_h = dt_over_h;
+  const size_t fbytes = (size_t)P.field_elems * gr::elem_size(P.dtype);
+  auto field = [&](void **p) -> hipError_t {
+    hipError_t e = hipMalloc(p, fbytes);
+    if (e == hipSuccess) e = hipMemset(*p, 0, fbytes);  // girih's zero halo, never written
+    if (e == hipSuccess) s->allocs.push_back(*p);
+    return e;
+  };
+  hipError_t e = hipSuccess;
+  for (int i = 0; i < 2 * s->nck && e == hipSuccess; ++i) {
+    void *p = nullptr;
+    if ((e = field(&p)) == hipSuccess) s->ck.push_back(p);
+  }
+  for (int i = 0; i < s->ckpt + 2 && e == hipSuccess; ++i) {
+    void *p = nullptr;
+    if ((e = field(&p)) == hipSuccess) s->seg.push_back(p);
+  }
+  for (int i = 0; i < 2 && e == hipSuccess; ++i) e = field(&s->nu[i]);
+  for (int i = 0; i < 2 && e == hipSuccess; ++i) e = field(&s->x[i]);
+  if (e == hipSuccess) {
+    e = hipMalloc(&s->g, sizeof(double) * (size_t)P.nx * P.ny * P.nz);
+    if (e == hipSuccess) s->allocs.push_back(s->g);
+  }
+  if (e != hipSuccess) {
+    gr_shot_destroy(s);
+    return gr::set_errorf(GR_ENOMEM, "gr_shot_create: device allocation failed: %s", hipGetErrorString(e));
+  }
+  *out = s;
+  return GR_OK;
+}
+
+extern "C" int gr_set_roc2_from_slowness2(gr_handle *h, const void *m_dev, double dt_over_h) {
+  if (!h || !m_dev) return gr::set_error(GR_EINVAL, "gr_set_roc2_from_slowness2: null argument");
+  const Problem &P = gr_internal_problem(h);
+  if (P.kernel != 0 || !P.roc2) return gr::set_error(GR_EINVAL, "gr_set_roc2_from_slowness2: no roc2 field");
+  if (!(dt_over_h > 0)) return gr::set_error(GR_EINVAL, "gr_set_roc2_from_slowness2: dt_over_h must be > 0");
+  const long long total = (long long)(P.nx + 2 * P.R) * P.rows * P.planes;
+  const unsigned nb = (unsigned)((total + 255) / 256);
+  const double k2 = dt_over_h * dt_over_h;
+  hipStream_t st = gr_internal_stream(h);
+  if (P.dtype == GR_F64)
+    hipLaunchKernelGGL((roc2_from_m_kernel<double, double>), dim3(nb), dim3(256), 0, st, (double *)P.roc2,
+                       (const double *)m_dev, P.nx, P.ny, P.nz, P.R, (long long)P.pitch, (long long)P.rows, total, k2);
+  else if (P.flags & GR_COEF_BF16)
+    hipLaunchKernelGGL((roc2_from_m_kernel<float, __hip_bfloat16>), dim3(nb), dim3(256), 0, st,
+                       (__hip_bfloat16 *)P.roc2, (const float *)m_dev, P.nx, P.ny, P.nz, P.R, (long long)P.pitch,
+                       (long long)P.rows, total, k2);
+  else
+    hipLaunchKernelGGL((roc2_from_m_kernel<float, float>), dim3(nb), dim3(256), 0, st, (float *)P.roc2,
+                       (const float *)m_dev, P.nx, P.ny, P.nz, P.R, (long long)P.pitch, (long long)P.rows, total, k2);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  if (e != hipSuccess) return shot_fail(e, "gr_set_roc2_from_slowness2");
+  return GR_OK;
+}
+
+extern "C" int gr_shot_forward(gr_shot *s, void *rec_dev) {
+  if (!s || !rec_dev) return gr::set_error(GR_EINVAL, "gr_shot_forward: null argument");
+  gr_handle *h = s->h;
+  const Problem &P = gr_internal_problem(h);
+  const gr::LaunchCfg &lc = gr_internal_launch_cfg(h);
+  hipStream_t st = gr_internal_stream(h);
+  const size_t fbytes = (size_t)P.field_elems * gr::elem_size(P.dtype);
+  const size_t plane = (size_t)P.nx * P.ny * gr::elem_size(P.dtype);
+  hipError_t e = hipSuccess;
+  for (int i = 0; i < 2 && e == hipSuccess; ++i) e = hipMemsetAsync(s->x[i], 0, fbytes, st);
+  void *prev = s->x[0], *cur = s->x[1];  // x_a, x_{a+1}
+  for (int a = 0; a < s->nt && e == hipSuccess; ++a) {
+    if (a % s->ckpt == 0) {
+      const int c = a / s->ckpt;
+      e = hipMemcpyAsync(s->ck[2 * c], prev, fbytes, hipMemcpyDeviceToDevice, st);
+      if (e == hipSuccess) e = hipMemcpyAsync(s->ck[2 * c + 1], cur, fbytes, hipMemcpyDeviceToDevice, st);
+      if (e != hipSuccess) break;
+    }
+    long long sidx = -1;
+    const void *sv = src_of(h, a, &sidx);
+    e = gr::sweep_fields(P, fbase(P, prev), fbase(P, cur), sv, sidx, lc, st);  // prev <- x_{a+2}
+    if (e == hipSuccess) e = record_any(P, fbase(P, prev), s->rec_k, (char *)rec_dev + (size_t)a * plane, st);
+    std::swap(prev, cur);
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  if (e != hipSuccess) return shot_fail(e, "gr_shot_forward");
+  s->have_forward = true;
+  return GR_OK;
+}
+
+extern "C" int gr_shot_adjoint(gr_shot *s, const void *res_dev, void *grad_dev) {
+  if (!s || !res_dev || !grad_dev) return gr::set_error(GR_EINVAL, "gr_shot_adjoint: null argument");
+  if (!s->have_forward) return gr::set_error(GR_EINVAL, "gr_shot_adjoint: run gr_shot_forward first");
+  gr_handle *h = s->h;
+  const Problem &P = gr_internal_problem(h);
+  const gr::LaunchCfg &lc = gr_internal_launch_cfg(h);
+  hipStream_t st = gr_internal_stream(h);
+  const size_t fbytes = (size_t)P.field_elems * gr::elem_size(P.dtype);
+  const size_t plane = (size_t)P.nx * P.ny * gr::elem_size(P.dtype);
+  const long long ncell = (long long)P.nx * P.ny * P.nz;
+  hipError_t e = hipSuccess;
+  for (int i = 0; i < 2 && e == hipSuccess; ++i) e = hipMemsetAsync(s->nu[i], 0, fbytes, st);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(zero_kernel, dim3((unsigned)((ncell + 255) / 256)), dim3(256), 0, st, s->g, ncell);
+    e = hipGetLastError();
+  }
+  void *n0 = s->nu[0], *n1 = s->nu[1];  // nu_{a+4}, nu_{a+3}
+  for (int c = s->nck - 1; c >= 0 && e == hipSuccess; --c) {
+    const int a0 = c * s->ckpt, a1 = std::min(a0 + s->ckpt, s->nt);
+    // recompute x_{a0} .. x_{a1+1} into seg[0 .. a1-a0+1]
+    e = hipMemcpyAsync(s->seg[0], s->ck[2 * c], fbytes, hipMemcpyDeviceToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(s->seg[1], s->ck[2 * c + 1], fbytes, hipMemcpyDeviceToDevice, st);
+    for (int a = a0; a < a1 && e == hipSuccess; ++a) {
+      const int t = a - a0 + 2;
+      long long sidx = -1;
+      const void *sv = src_of(h, a, &sidx);
+      e = gr::sweep_fields(P, fbase(P, s->seg[t]), fbase(P, s->seg[t - 1]), sv, sidx, lc, st,
+                           fbase(P, s->seg[t - 2]));
+    }
+    for (int a = a1 - 1; a >= a0 && e == hipSuccess; --a) {
+      const int t = a - a0 + 2;
+      e = gr::sweep_fields(P, fbase(P, n0), fbase(P, n1), nullptr, -1, lc, st);  // n0 <- nu_{a+2}
+      if (e == hipSuccess) e = inject_any(P, fbase(P, n0), s->rec_k, (const char *)res_dev + (size_t)a * plane, st);
+      long long sidx = -1;
+      const void *sv = src_of(h, a, &sidx);
+      if (e == hipSuccess)
+        e = grad_any(P, s->g, fbase(P, n0), fbase(P, s->seg[t]), fbase(P, s->seg[t - 1]), fbase(P, s->seg[t - 2]),
+                     sv, sidx, st);
+      std::swap(n0, n1);
+    }
+  }
+  if (e == hipSuccess) {
+    const double scale = -1.0 / (s->dt_over_h * s->dt_over_h);
+    const unsigned nb = (unsigned)((ncell + 255) / 256);
+    if (P.dtype == GR_F64)
+      hipLaunchKernelGGL(grad_final_kernel<double>, dim3(nb), dim3(256), 0, st, s->g, (double *)grad_dev, ncell, scale);
+    else
+      hipLaunchKernelGGL(grad_final_kernel<float>, dim3(nb), dim3(256), 0, st, s->g, (float *)grad_dev, ncell, scale);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  if (e != hipSuccess) return shot_fail(e, "gr_shot_adjoint");
+  return GR_OK;
+}

@@ -72,7 +72,7 @@ __device__ __forceinline__ X uniform_load(const X *p) {
 template <typename C, int BY, int RY, bool RING>
 __global__ __launch_bounds__(64 * BY) void batch_kernel(const BatchDesc *__restrict__ d, int nprob, int nitems,
                                                         unsigned long long *ctr, unsigned long long base, int step) {
-  using ST = SweepTile<float, C, 0, BY, RY, true, false, RING, 2>;
+  using ST = SweepTile<float, C, 0, BY, RY, true, false, RING, kFastND>;
   __shared__ typename ST::Lds lds;
   __shared__ int sclaim[2];
   const bool leader = threadIdx.x == 0 && threadIdx.y == 0;
@@ -193,7 +193,7 @@ int build_tables(gr_batch *b, bool spares) {
     item += cnt;
     D.roc2 = P.roc2;
     for (int r = 0; r < 5; ++r) D.c[r] = (float)P.c[r];
-    D.csum = (float)(P.c[0] + 6.0 * (P.c[1] + P.c[2]));
+    D.csum = fast_csum(P.c);
     const int par = gr_internal_parity(h);
     int nsrc = 0;
     long long sidx = -1;

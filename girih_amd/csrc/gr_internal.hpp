@@ -32,8 +32,9 @@ void tile_of_variant(int kid, int dtype, int variant, int *tx, int *ty);
 size_t elem_size(int dtype);
 int set_error(int code, const char *msg);
 int set_errorf(int code, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
+// One sweep u <- f(v); kernel 0 reads u^(n-1) from uold when given (else from u).
 hipError_t sweep_fields(const Problem &P, void *u, const void *v, const void *src, long long src_idx,
-                        const LaunchCfg &lc, hipStream_t st);
+                        const LaunchCfg &lc, hipStream_t st, const void *uold = nullptr);
 hipError_t tblock_pass(const Problem &P, int tl, const void *X, const void *Y, void *ZO, void *ZN, const void *src,
                        long long src_idx, const LaunchCfg &lc, hipStream_t st);
 int tb_depths(int kid, int dtype, int *out, int max);
@@ -50,4 +51,5 @@ int gr_internal_parity(const gr_handle *h);
 long long gr_internal_napplied(const gr_handle *h);
 const void *gr_internal_source(const gr_handle *h, int *n, long long *idx);
 hipStream_t gr_internal_stream(const gr_handle *h);
+const gr::LaunchCfg &gr_internal_launch_cfg(const gr_handle *h);
 void gr_internal_advance(gr_handle *h, int napply, double ms);

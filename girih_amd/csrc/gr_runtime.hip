@@ -137,7 +137,7 @@ static hipError_t launch_sweep_k(int kid, bool fast, bool rocs, const SweepArgs<
 
 // One sweep u <- f(v) with the handle's configuration.
 hipError_t sweep_fields(const Problem &P, void *u, const void *v, const void *src, long long src_idx,
-                        const LaunchCfg &lc, hipStream_t st) {
+                        const LaunchCfg &lc, hipStream_t st, const void *uold) {
   const bool fast = (P.flags & GR_FAST) != 0;
   const bool rocs = (P.flags & GR_ROC2_SCALAR) != 0;
   const bool bf16 = (P.flags & GR_COEF_BF16) != 0;
@@ -154,6 +154,7 @@ hipError_t sweep_fields(const Problem &P, void *u, const void *v, const void *sr
     SweepArgs<double, double> a{};
     a.u = (double *)u;
     a.v = (const double *)v;
+    a.uold = (const double *)uold;
     a.roc2 = (const double *)P.roc2;
     a.coef = (const double *)P.coef;
     a.cstride = P.field_elems;
@@ -169,11 +170,12 @@ hipError_t sweep_fields(const Problem &P, void *u, const void *v, const void *sr
     SweepArgs<float, __hip_bfloat16> a{};
     a.u = (float *)u;
     a.v = (const float *)v;
+    a.uold = (const float *)uold;
     a.roc2 = (const __hip_bfloat16 *)P.roc2;
     a.coef = (const __hip_bfloat16 *)P.coef;
     a.cstride = P.field_elems;
     for (int r = 0; r < 5; ++r) a.c[r] = (float)P.c[r];
-    a.csum = (float)(P.c[0] + 6.0 * (P.c[1] + P.c[2]));
+    a.csum = fast_csum(P.c);
     a.roc2s = (float)P.roc2s;
     a.g = g;
     a.src = (const float *)src;
@@ -184,16 +186,17 @@ hipError_t sweep_fields(const Problem &P, void *u, const void *v, const void *sr
   SweepArgs<float, float> a{};
   a.u = (float *)u;
   a.v = (const float *)v;
+  a.uold = (const float *)uold;
   a.roc2 = (const float *)P.roc2;
   a.coef = (const float *)P.coef;
   a.cstride = P.field_elems;
   for (int r = 0; r < 5; ++r) a.c[r] = (float)P.c[r];
-  a.csum = (float)(P.c[0] + 6.0 * (P.c[1] + P.c[2]));
+  a.csum = fast_csum(P.c);
   a.roc2s = (float)P.roc2s;
   a.g = g;
   a.src = (const float *)src;
   a.src_k = src_idx < 0 ? -1 : (int)(src_idx / g.sz);
-    a.src_off = src_idx < 0 ? 0 : (int)(src_idx % g.sz);
+  a.src_off = src_idx < 0 ? 0 : (int)(src_idx % g.sz);
   return launch_sweep_k<float, float>(P.kernel, fast, rocs, a, lc, st);
 }
 
@@ -1035,6 +1038,7 @@ const void *gr_internal_source(const gr_handle *h, int *n, long long *idx) {
   return h->src;
 }
 hipStream_t gr_internal_stream(const gr_handle *h) { return h->stream; }
+const gr::LaunchCfg &gr_internal_launch_cfg(const gr_handle *h) { return h->lc; }
 void gr_internal_advance(gr_handle *h, int napply, double ms) {
   h->parity ^= napply & 1;
   h->napplied += napply;

@@ -30,7 +30,21 @@
 #include <hip/hip_bf16.h>
 #include <stdint.h>
 
+#ifndef GR_FAST_ND
+#define GR_FAST_ND 2
+#endif
+
 namespace gr {
+
+// FAST kernel 0: radii 1..kFastND use the difference form (see Op<T, 0, true>).
+constexpr int kFastND = GR_FAST_ND;
+
+// Centre weight of the FAST Laplacian, c0 + 6*sum_{r <= kFastND} c_r, rounded once from FP64.
+inline float fast_csum(const double *c) {
+  double s = c[0];
+  for (int r = 1; r <= kFastND; ++r) s += 6.0 * c[r];
+  return (float)s;
+}
 
 __host__ __device__ constexpr int kradius(int kid) { return (kid == 0 || kid == 4) ? 4 : 1; }
 __host__ __device__ constexpr int kncoef(int kid) {
@@ -46,13 +60,14 @@ struct Geom {
 
 template <typename T, typename C>
 struct SweepArgs {
-  T *u;                // output (kernel 0: holds u^(n-1) on entry)
+  T *u;                // output (kernel 0: holds u^(n-1) on entry unless uold is set)
   const T *v;          // input field
+  const T *uold;       // kernel 0: u^(n-1) when it lives in another field than u (null = u)
   const C *roc2;       // kernel 0 roc2 field (unless scalar)
   const C *coef;       // coefficient arrays, array m at coef + m*cstride
   long long cstride;
   T c[5];              // constant coefficients
-  T csum;              // FAST kernel 0 centre weight c0 + 6*(c1+c2), rounded once from FP64
+  T csum;              // FAST kernel 0 centre weight (fast_csum)
   T roc2s;             // scalar roc2
   Geom g;
   int tiles_x, tiles_y, nzc, zchunk;
@@ -189,7 +204,7 @@ struct SweepTile;
 // fetched in the same step as the interior of that plane (R+1 planes ahead), when
 // the neighbouring tiles fetch it too and it is still in L2; otherwise the LDS is
 // double buffered and the halo is fetched one plane ahead.
-template <typename T, typename C, int KID, int BY, int RY, bool FAST, bool ROCS, int MINW, bool RING, int ND = 2>
+template <typename T, typename C, int KID, int BY, int RY, bool FAST, bool ROCS, int MINW, bool RING, int ND = kFastND>
 __global__ __launch_bounds__(64 * BY) __attribute__((amdgpu_waves_per_eu(MINW > 0 ? MINW : 1)))
 void sweep_kernel(const SweepArgs<T, C> a) {
   using ST = SweepTile<T, C, KID, BY, RY, FAST, ROCS, RING, ND>;
@@ -311,7 +326,7 @@ __device__ __forceinline__ void SweepTile<T, C, KID, BY, RY, FAST, ROCS, RING, N
       T uo[RY], r2[RY];
       T cf[RY][NC > 0 ? NC : 1];
       {
-        const T *up = plane(a.u, k);
+        const T *up = plane(a.uold ? a.uold : a.u, k);
         const long long pk = (long long)k * g.sz;
 #pragma unroll
         for (int m = 0; m < RY; ++m) {

@@ -227,7 +227,7 @@ __global__ __launch_bounds__(NT) void tb_kernel(const TbArgs<T> a) {
               if constexpr (KID == 0) {
                 const T roc = ROCS ? a.roc2s : a.roc2[(long long)p * a.sz + off[j] / sizeof(T)];
                 out = Op<T, 0, FAST>::update(v0, uo[j], roc,
-                                             Op<T, 0, FAST>::template lap<2>(c, a.csum, v0, zq, xn, yn));
+                                             Op<T, 0, FAST>::template lap<kFastND>(c, a.csum, v0, zq, xn, yn));
               } else if constexpr (KID == 1) {  // stencils.c:73-78
                 out = c[0] * v0 + c[1] * (xn(1) + xn(-1)) + c[1] * (yn(1) + yn(-1)) + c[1] * (zq[2] + zq[0]);
               } else {  // KID == 5, stencils.c:193-201
@@ -369,7 +369,7 @@ hipError_t tblock_pass(const Problem &P, int tl, const void *X, const void *Y, v
   a.coef = (const float *)P.coef;
   a.cstride = P.field_elems;
   for (int r = 0; r < 5; ++r) a.c[r] = (float)P.c[r];
-  a.csum = (float)(P.c[0] + 6.0 * (P.c[1] + P.c[2]));
+  a.csum = fast_csum(P.c);
   a.roc2s = (float)P.roc2s;
   a.sy = P.pitch;
   a.sz = sz;

@@ -70,7 +70,8 @@ EXPORTS = ["gr_create", "gr_destroy", "gr_last_error", "gr_upload", "gr_download
            "gr_init_coef_uniform", "gr_compare_fields", "gr_set_config", "gr_set_xcd_order", "gr_autotune",
            "gr_init_roc2_model",
            "gr_download_roc2", "gr_batch_create", "gr_batch_destroy", "gr_batch_problem", "gr_batch_set_config",
-           "gr_batch_step", "gr_batch_autotune", "gr_batch_get_stats"]
+           "gr_batch_step", "gr_batch_autotune", "gr_batch_get_stats", "gr_shot_create", "gr_shot_destroy",
+           "gr_set_roc2_from_slowness2", "gr_shot_forward", "gr_shot_adjoint"]
 
 _lib = None
 
@@ -119,6 +120,11 @@ def lib():
             "gr_batch_step": ([P, I, I], I),
             "gr_batch_autotune": ([P, I, I, IP, IP], I),
             "gr_batch_get_stats": ([P, ctypes.POINTER(BatchStats)], I),
+            "gr_shot_create": ([P, I, I, I, ctypes.c_double, ctypes.POINTER(P)], I),
+            "gr_shot_destroy": ([P], I),
+            "gr_set_roc2_from_slowness2": ([P, P, ctypes.c_double], I),
+            "gr_shot_forward": ([P, P], I),
+            "gr_shot_adjoint": ([P, P, P], I),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)

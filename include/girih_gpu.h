@@ -225,6 +225,27 @@ int gr_batch_step(gr_batch *b, int napply, int use_graph);
 int gr_batch_autotune(gr_batch *b, int steps, int use_graph, int *variant, int *zchunk);
 int gr_batch_get_stats(gr_batch *b, gr_batch_stats *out);
 
+/* ---- Shots: receiver traces and the adjoint gradient (no counterpart in girih,
+ * whose point source is the only I/O, nb_naive_ts.c:190,196) ----
+ * A shot runs kernel 0 on handle h (per-cell roc2 field; GR_F64, or GR_F32 with
+ * GR_FAST) from zero fields for nt applications with h's point source
+ * (gr_set_source), recording the interior of plane rec_k (halo-inclusive z index)
+ * after every application.  The gradient is taken with respect to the squared
+ * slowness m = 1/v^2, roc2 = dt_over_h^2 / m, by the discrete adjoint of the
+ * stepper with checkpoints every ckpt applications and recomputation.
+ * Device arrays are dense, x fastest, in the field type:
+ *   m, gradient [nz][ny][nx];  traces, cotangents [nt][ny][nx].
+ * All calls are synchronous on the handle's stream. */
+typedef struct gr_shot gr_shot;
+int gr_shot_create(gr_handle *h, int rec_k, int nt, int ckpt, double dt_over_h, gr_shot **out);
+int gr_shot_destroy(gr_shot *s);
+/* roc2 = dt_over_h^2 / m on the whole field (halo cells copy the nearest interior m). */
+int gr_set_roc2_from_slowness2(gr_handle *h, const void *m_dev, double dt_over_h);
+/* Forward run: traces into trace_dev; keeps the checkpoints for gr_shot_adjoint. */
+int gr_shot_forward(gr_shot *s, void *trace_dev);
+/* Vector-Jacobian product of the last forward run: grad = sum_a res[a] . d trace[a] / dm. */
+int gr_shot_adjoint(gr_shot *s, const void *res_dev, void *grad_dev);
+
 #ifdef __cplusplus
 }
 #endif

@@ -72,6 +72,17 @@ int gro_compare_f64(const double *u, const double *target, int nx, int ny, int n
 int gro_compare_f32(const float *u, const float *target, int nx, int ny, int nz, int R,
                     double *emax, double *el1);
 
+/*
+ * Seismic shot on kernel 0 in FP64 (adjoint.c; no girih counterpart): forward
+ * traces of plane rec_k (trace [nt][ny][nx], may be NULL) and, when res and grad
+ * are given, the adjoint gradient of sum_a <res[a], trace[a]> with respect to the
+ * squared slowness m = dt_over_h^2 / roc2 (grad [nz][ny][nx]).  Zero initial
+ * fields; src_idx < 0 for no source.  Returns 0, or < 0 on bad arguments.
+ */
+int gro_shot_f64(const int shape[3], const double coef[5], const double *roc2, int nt, int64_t src_idx,
+                 const double *src, int rec_k, double *trace, const double *res, double *grad,
+                 double dt_over_h);
+
 #ifdef __cplusplus
 }
 #endif

@@ -53,6 +53,8 @@ def lib():
             f.restype = None
         L.gro_mwd_round_nt.argtypes = [I, I]
         L.gro_mwd_round_nt.restype = I
+        L.gro_shot_f64.argtypes = [P, P, P, I, ctypes.c_int64, P, I, P, P, P, ctypes.c_double]
+        L.gro_shot_f64.restype = I
         _lib = L
     return _lib
 
@@ -154,3 +156,33 @@ def to_padded(a, kernel, alignment=8):
     out = np.zeros((nnz, nny, nnx + pad), dtype=a.dtype)
     out[:, :, :nnx] = a
     return out
+
+
+def shot(coef, roc2, nt, src_idx, src, rec_k, res=None, dt_over_h=1.0):
+    """Seismic shot on kernel 0 in FP64 (adjoint.c): zero initial fields, point
+    source src[a] at halo-inclusive (i, j, k) = src_idx (or None), traces of plane
+    rec_k.  Returns (trace [nt, ny, nx], grad [nz, ny, nx] or None); grad is the
+    gradient of sum_a <res[a], trace[a]> with respect to m = dt_over_h**2 / roc2."""
+    roc2 = np.ascontiguousarray(roc2, dtype=np.float64)
+    nnz, nny, nnx = roc2.shape
+    nx, ny, nz = nnx - 8, nny - 8, nnz - 8
+    shape = (ctypes.c_int * 3)(nnx, nny, nnz)
+    c = np.zeros(5)
+    c[:] = np.asarray(coef, dtype=np.float64)[:5]
+    flat = -1
+    if src_idx is not None:
+        i, j, k = src_idx
+        flat = (k * nny + j) * nnx + i
+        src = np.ascontiguousarray(src, dtype=np.float64)
+        assert src.size >= nt
+    trace = np.zeros((nt, ny, nx))
+    grad = None
+    if res is not None:
+        res = np.ascontiguousarray(res, dtype=np.float64)
+        assert res.shape == (nt, ny, nx)
+        grad = np.zeros((nz, ny, nx))
+    rc = lib().gro_shot_f64(shape, _ptr(c), _ptr(roc2), int(nt), flat, _ptr(src) if flat >= 0 else None, int(rec_k),
+                            _ptr(trace), _ptr(res), _ptr(grad), float(dt_over_h))
+    if rc != 0:
+        raise ValueError("gro_shot_f64 failed (%d)" % rc)
+    return trace, grad
