@@ -17,7 +17,9 @@ Prints ONE JSON line (N = 1).  --gpus N > 1 is refused: this build runs on one G
 use the configuration the tuned run printed).  The "extras" object measures the
 other workloads of the same stepper: the 7-point Jacobi at 128^3 (FP64), the
 7-point variable-coefficient stencil at 512^3 (FP64 and FP32, with the FP32
-error), and the batch of 16 uneven 25-point problems with bf16 roc2.
+error), the batch of 16 uneven 25-point problems with bf16 roc2, and the adjoint
+gradient of a 400^3 x 2000-step shot (FP32, checkpointed) with its FP64 and
+finite-difference checks.
 """
 import argparse
 import json
@@ -153,6 +155,12 @@ def extra_first_order(kernel, n, nt, coef=None, fp32=True):
     if "f32" in keep:
         c = keep["f32"].compare(1, keep["f64"], 1)
         out["f32"]["rel_l2_vs_f64"] = c["rel_l2"]
+        out["f32"]["f64_max_abs_after_run"] = c["max_ref"]  # below FP32's range when the stencil contracts
+        for st in keep.values():  # the error while the field is still representable in FP32
+            st.init_fields(1, 12345)
+            st.step(100)
+        c = keep["f32"].compare(1, keep["f64"], 1)
+        out["f32"]["rel_l2_vs_f64_after_100"] = c["rel_l2"]
     for st in keep.values():
         st.close()
     return out
@@ -207,6 +215,92 @@ def extra_batch(nt, seed=2024):
             "graph_first_call_ms": round(t_first, 2),
             "sweep_roofline_frac": round(gl * 14 / HBM_PEAK_GBS, 4), "bytes_per_lup_sweep": 14,
             "rel_l2_vs_f64_max": max(errs), "rel_l2_vs_f64": [float("%.3g" % e) for e in errs]}
+
+
+def extra_adjoint(n=400, nt=2000, ckpt=50, nfd=20, seed=7):
+    """Adjoint gradient of the 25-pt variable-velocity stepper: n^3, nt steps,
+    h = 10 m, 8-layer velocity (+-5% blobs), 15 Hz Ricker at 50 m depth, receiver
+    plane at z = 8 cells every step, misfit 0.5 ||d_pred - d_obs||^2 with d_obs from
+    a +3% Gaussian-blob velocity perturbation; FP32 shot with checkpoints every
+    `ckpt` steps through torch.autograd, checked against the FP64 GPU adjoint
+    (bit-identical to the FP64 CPU adjoint at test sizes) and against central
+    finite differences of the FP64 forward model on `nfd` cells the wave reached."""
+    import numpy as np
+    import torch
+    from girih_amd import adjoint, gpu, physics
+    h = 10.0
+    dt = physics.cfl_dt(4500.0 * 1.05, h, 0.4)
+    dth = dt / h
+    src = physics.roc2_const(1500.0, dt, h) * physics.ricker(15.0, dt, nt)
+    src_pt = (4 + n // 2, 4 + n // 2, 4 + 5)
+    rec_k = 4 + 8
+
+    def slowness2(anomaly):
+        with gpu.Stepper(0, n, n, n, dtype=gpu.GR_F64, coef=physics.fd8_coef()) as st:
+            st.init_roc2_model(gpu.vmodel(seed, dth, anomaly=anomaly, anomaly_center=(n / 2, n / 2, n / 2),
+                                          anomaly_sigma=n / 10.0))
+            r2 = st.download_roc2()[4:-4, 4:-4, 4:4 + n]
+        return np.ascontiguousarray(dth ** 2 / r2)
+
+    m_obs, m0 = slowness2(0.03), slowness2(0.0)
+    s64 = adjoint.Shot(n, n, n, nt, dth, src_pt, src, rec_k, ckpt=ckpt, dtype=gpu.GR_F64)
+    s32 = adjoint.Shot(n, n, n, nt, dth, src_pt, src.astype(np.float32).astype(np.float64), rec_k, ckpt=ckpt,
+                       dtype=gpu.GR_F32)
+    try:
+        d_obs = s64.forward(torch.tensor(m_obs, device="cuda"))
+        m64 = torch.tensor(m0, device="cuda")
+        tr64 = s64.forward(m64)
+        g64 = s64.vjp(m64, tr64 - d_obs)
+        # FP32: forward + backward through the autograd Function, timed
+        m32 = torch.tensor(m0, dtype=torch.float32, device="cuda", requires_grad=True)
+        _sync()
+        t0 = time.perf_counter()
+        tr32 = adjoint.shot_traces(m32, s32)
+        _sync()
+        t1 = time.perf_counter()
+        loss = 0.5 * ((tr32.double() - d_obs) ** 2).sum()
+        loss.backward()
+        _sync()
+        t2 = time.perf_counter()
+        g32 = m32.grad.double()
+        rel = float(torch.linalg.norm(g32 - g64) / torch.linalg.norm(g64))
+        J0 = float(0.5 * ((tr64 - d_obs) ** 2).sum())
+        # the FP32 gradient error follows from the FP32 traces' error relative to the residual
+        tr_err = float(torch.linalg.norm(tr32.double() - tr64) / torch.linalg.norm(tr64))
+        res_rel = float(torch.linalg.norm(tr64 - d_obs) / torch.linalg.norm(tr64))
+        del tr32, tr64
+        g = g64.cpu().numpy()
+        g32h = g32.cpu().numpy()
+        gmax = np.abs(g).max()
+        live = np.argwhere(np.abs(g) > 1e-2 * gmax)
+        rng = np.random.default_rng(seed)
+        errs32, errs64, errs32n = [], [], []
+
+        def J(mm):
+            t = s64.forward(torch.tensor(mm, device="cuda"))
+            return 0.5 * float(((t - d_obs) ** 2).sum())
+
+        for q in rng.choice(len(live), min(nfd, len(live)), replace=False):
+            k, j, i = live[q]
+            hh = 1e-4 * m0[k, j, i]
+            mp, mm = m0.copy(), m0.copy()
+            mp[k, j, i] += hh
+            mm[k, j, i] -= hh
+            fd = (J(mp) - J(mm)) / (2 * hh)
+            errs32.append(abs(g32h[k, j, i] - fd) / abs(fd))
+            errs64.append(abs(g[k, j, i] - fd) / abs(fd))
+            errs32n.append(abs(g32h[k, j, i] - fd) / gmax)
+    finally:
+        s32.close()
+        s64.close()
+    fwd_glups = n ** 3 * nt / (t1 - t0) / 1e9
+    return {"grid": n, "steps": nt, "checkpoint_every": ckpt, "forward_s": round(t1 - t0, 3),
+            "backward_s": round(t2 - t1, 3), "forward_glups": round(fwd_glups, 2),
+            "backward_over_forward": round((t2 - t1) / (t1 - t0), 3), "misfit": J0,
+            "rel_l2_grad_fp32_vs_fp64": rel, "fd_cells": len(errs32),
+            "fd_max_rel_err_fp32_adjoint": max(errs32), "fd_max_rel_err_fp64_adjoint": max(errs64),
+            "fd_max_err_fp32_adjoint_over_max_grad": max(errs32n),
+            "rel_l2_traces_fp32_vs_fp64": tr_err, "residual_over_traces": res_rel}
 
 
 def main():
@@ -294,6 +388,7 @@ def main():
             "jacobi7_f64_128": extra_first_order(1, 128, 100, coef=[0.4, 0.1], fp32=False),
             "var7_nosym_512": extra_first_order(5, 512, 500),
             "batch16_wave25_bf16": extra_batch(500),
+            "adjoint_wave25_400": extra_adjoint(),
         }
     if not args.no_cpu_baseline:
         print("bench: cpu baseline", file=sys.stderr, flush=True)

@@ -18,7 +18,8 @@
 //
 // The forward pass stores (x_{cK}, x_{cK+1}) every K applications; the backward
 // pass recomputes each segment's K states from its checkpoint into a buffer and
-// walks it backwards.  Memory: 2*ceil(nt/K) + K + 4 fields + an FP64 accumulator.
+// walks it backwards.  Memory: 2*ceil(nt/K) + 2 fields for the forward pass;
+// the first backward pass adds K + 4 fields and an FP64 accumulator.
 #include <hip/hip_runtime.h>
 #include <hip/hip_bf16.h>
 
@@ -216,16 +217,7 @@ extern "C" int gr_shot_create(gr_handle *h, int rec_k, int nt, int ckpt, double 
     void *p = nullptr;
     if ((e = field(&p)) == hipSuccess) s->ck.push_back(p);
   }
-  for (int i = 0; i < s->ckpt + 2 && e == hipSuccess; ++i) {
-    void *p = nullptr;
-    if ((e = field(&p)) == hipSuccess) s->seg.push_back(p);
-  }
-  for (int i = 0; i < 2 && e == hipSuccess; ++i) e = field(&s->nu[i]);
   for (int i = 0; i < 2 && e == hipSuccess; ++i) e = field(&s->x[i]);
-  if (e == hipSuccess) {
-    e = hipMalloc(&s->g, sizeof(double) * (size_t)P.nx * P.ny * P.nz);
-    if (e == hipSuccess) s->allocs.push_back(s->g);
-  }
   if (e != hipSuccess) {
     gr_shot_destroy(s);
     return gr::set_errorf(GR_ENOMEM, "gr_shot_create: device allocation failed: %s", hipGetErrorString(e));
@@ -300,6 +292,32 @@ extern "C" int gr_shot_adjoint(gr_shot *s, const void *res_dev, void *grad_dev) 
   const size_t plane = (size_t)P.nx * P.ny * gr::elem_size(P.dtype);
   const long long ncell = (long long)P.nx * P.ny * P.nz;
   hipError_t e = hipSuccess;
+  if (s->seg.empty()) {  // backward-pass buffers, allocated on first use (forward-only shots never hold them)
+    const size_t before = s->allocs.size();
+    auto field = [&](void **p) -> hipError_t {
+      hipError_t r = hipMalloc(p, fbytes);
+      if (r == hipSuccess) r = hipMemset(*p, 0, fbytes);  // zero halo, never written
+      if (r == hipSuccess) s->allocs.push_back(*p);
+      return r;
+    };
+    for (int i = 0; i < s->ckpt + 2 && e == hipSuccess; ++i) {
+      void *p = nullptr;
+      if ((e = field(&p)) == hipSuccess) s->seg.push_back(p);
+    }
+    for (int i = 0; i < 2 && e == hipSuccess; ++i) e = field(&s->nu[i]);
+    if (e == hipSuccess) {
+      e = hipMalloc(&s->g, sizeof(double) * (size_t)ncell);
+      if (e == hipSuccess) s->allocs.push_back(s->g);
+    }
+    if (e != hipSuccess) {  // release the partial set so a later call starts clean
+      for (size_t i = before; i < s->allocs.size(); ++i) (void)hipFree(s->allocs[i]);
+      s->allocs.resize(before);
+      s->seg.clear();
+      s->nu[0] = s->nu[1] = nullptr;
+      s->g = nullptr;
+      return gr::set_errorf(GR_ENOMEM, "gr_shot_adjoint: device allocation failed: %s", hipGetErrorString(e));
+    }
+  }
   for (int i = 0; i < 2 && e == hipSuccess; ++i) e = hipMemsetAsync(s->nu[i], 0, fbytes, st);
   if (e == hipSuccess) {
     hipLaunchKernelGGL(zero_kernel, dim3((unsigned)((ncell + 255) / 256)), dim3(256), 0, st, s->g, ncell);

@@ -7,7 +7,7 @@
   independent of the checkpoint interval (bit for bit);
 * torch.autograd: loss.backward() through the Function gives the adjoint
   gradient, which agrees with central finite differences of the FP64 GPU forward
-  model on random cells to 1e-3.
+  model to 1e-3 on 20 random cells the wave has reached.
 """
 import numpy as np
 import pytest
@@ -128,8 +128,9 @@ def test_autograd_function_and_finite_differences():
 
         rng = np.random.default_rng(11)
         gmax = np.abs(g).max()
-        for _ in range(20):
-            k, j, i = rng.integers(0, NZ), rng.integers(0, NY), rng.integers(0, NX)
+        live = np.argwhere(np.abs(g) > 1e-2 * gmax)  # cells the wave reached
+        for n in rng.choice(len(live), 20, replace=False):
+            k, j, i = live[n]
             h = 1e-4 * m[k, j, i]
             mp = m.copy()
             mm = m.copy()
