@@ -61,37 +61,53 @@ int set_errorf(int code, const char *fmt, ...) {
 size_t elem_size(int dtype) { return dtype == GR_F64 ? 8 : 4; }
 
 // ------------------------------------------------------------------ launch plumbing
-template <typename T, typename C, int KID, bool FAST, bool ROCS, int BY, int RY, int MINW, bool RING = false>
+template <typename T, typename C, int KID, bool FAST, bool ROCS, int BY, int RY, int MINW, bool RING = false, int BX = 1>
 static hipError_t launch_cfg(const SweepArgs<T, C> &a0, const LaunchCfg &lc, hipStream_t st) {
   SweepArgs<T, C> a = a0;
-  const int TY = BY * RY;
+  const int TX = 64 * BX, TY = BY * RY;
   const int bx = a.g.xe - a.g.xb, by = a.g.ye - a.g.yb, bz = a.g.ze - a.g.zb;
-  a.tiles_x = (bx + 63) / 64;
+  a.tiles_x = (bx + TX - 1) / TX;
   a.tiles_y = (by + TY - 1) / TY;
   a.zchunk = lc.zchunk > 0 ? std::min(lc.zchunk, bz) : bz;
   a.nzc = (bz + a.zchunk - 1) / a.zchunk;
   a.xcd_order = lc.xcd_order;
   const int nb = a.tiles_x * a.tiles_y * a.nzc;
-  hipLaunchKernelGGL((sweep_kernel<T, C, KID, BY, RY, FAST, ROCS, MINW, RING>), dim3(nb), dim3(64, BY), 0, st, a);
+  hipLaunchKernelGGL((sweep_kernel<T, C, KID, BY, RY, FAST, ROCS, MINW, RING, kFastND, BX>), dim3(nb), dim3(TX, BY), 0,
+                     st, a);
   return hipGetLastError();
 }
 
 // Tile variants selectable at run time for the kernels on the hot path (the
-// other kernels use their default tile):
-//   0: 64 x BY lanes x RY rows default   1: 64x32 (8 x 4)   2: 64x16 (8 x 2)
+// other kernels use their default tile).  Lanes are 64*BX x BY, RY rows per lane:
+//   0: default (64 x 16)                 1: 64x32 (8 x 4)   2: 64x16 (8 x 2)
 //   3: 64x32 (8 x 4), >= 4 (FP32) / 3 (FP64) waves per SIMD
 //   4: 64x16 (4 x 4), >= 4 (FP32) / 3 (FP64) waves per SIMD
 //   5: as 3 with the LDS plane ring (halo fetched with the plane interior)
 //   6: 64x16 (8 x 2) with the LDS plane ring
 //   7: as 4 with the LDS plane ring
+//   8: 128x16 (2 waves wide, 4 x 4)      9: 128x32 (8 x 4, 1024 lanes)
+//  10: as 8 with the LDS plane ring     11: as 8 with >= 4 / 3 waves per SIMD
+//  12: 128x8 (2 x 4) ring, 256 lanes    13: 128x16 (8 x 2) ring, 1024 lanes
+//  14: as 10 with >= 4 / 3 waves per SIMD
+//  15: 256x8 (4 waves wide, 2 x 4) ring (double-buffered tile where the ring does not fit in LDS)
 int num_tile_variants(int kid, int dtype) {
   (void)dtype;
-  return (kid == 0 || kid == 1 || kid == 5) ? 8 : 1;
+  return (kid == 0 || kid == 1 || kid == 5) ? 16 : 1;
 }
 void tile_of_variant(int kid, int dtype, int variant, int *tx, int *ty) {
-  static const int rows[8] = {16, 32, 16, 32, 16, 32, 16, 16};
-  *tx = 64;
-  *ty = (variant > 0 && variant < num_tile_variants(kid, dtype)) ? rows[variant] : 16;
+  static const int rows[16] = {16, 32, 16, 32, 16, 32, 16, 16, 16, 32, 16, 16, 8, 16, 16, 8};
+  static const int cols[16] = {64, 64, 64, 64, 64, 64, 64, 64, 128, 128, 128, 128, 128, 128, 128, 256};
+  const bool ok = variant > 0 && variant < num_tile_variants(kid, dtype);
+  *tx = ok ? cols[variant] : 64;
+  *ty = ok ? rows[variant] : 16;
+}
+
+// The LDS plane ring holds R+2 planes of the tile plus halo; where that exceeds the
+// 160 KB of LDS of a CU the variant falls back to the double-buffered tile.
+template <typename T, int KID, int BX, int BY, int RY>
+constexpr bool ring_fits() {
+  constexpr int R = kradius(KID);
+  return (size_t)(R + 2) * (BY * RY + 2 * R) * (64 * BX + 2 * R) * sizeof(T) <= 160 * 1024;
 }
 
 template <typename T, typename C, int KID, bool FAST, bool ROCS>
@@ -107,6 +123,14 @@ static hipError_t launch_sweep_t(const SweepArgs<T, C> &a, const LaunchCfg &lc, 
       case 5: return launch_cfg<T, C, KID, FAST, ROCS, 8, 4, W, true>(a, lc, st);
       case 6: return launch_cfg<T, C, KID, FAST, ROCS, 8, 2, 0, true>(a, lc, st);
       case 7: return launch_cfg<T, C, KID, FAST, ROCS, 4, 4, W, true>(a, lc, st);
+      case 8: return launch_cfg<T, C, KID, FAST, ROCS, 4, 4, 0, false, 2>(a, lc, st);
+      case 9: return launch_cfg<T, C, KID, FAST, ROCS, 8, 4, 0, false, 2>(a, lc, st);
+      case 10: return launch_cfg<T, C, KID, FAST, ROCS, 4, 4, 0, true, 2>(a, lc, st);
+      case 11: return launch_cfg<T, C, KID, FAST, ROCS, 4, 4, W, false, 2>(a, lc, st);
+      case 12: return launch_cfg<T, C, KID, FAST, ROCS, 2, 4, 0, true, 2>(a, lc, st);
+      case 13: return launch_cfg<T, C, KID, FAST, ROCS, 8, 2, 0, true, 2>(a, lc, st);
+      case 14: return launch_cfg<T, C, KID, FAST, ROCS, 4, 4, W, true, 2>(a, lc, st);
+      case 15: return launch_cfg<T, C, KID, FAST, ROCS, 2, 4, 0, ring_fits<T, KID, 4, 2, 4>(), 4>(a, lc, st);
       default: break;
     }
   }

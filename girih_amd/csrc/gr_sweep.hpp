@@ -5,8 +5,9 @@
 // FUNC_BODY operators of src/kernels/stencils.c:27-216).
 //
 // Scheme (2.5-D z streaming):
-//   * a workgroup of 64 x BY lanes owns an output tile of 64 (x) x BY*RY (y)
-//     cells and marches through a chunk of z planes;
+//   * a workgroup of TX = 64*BX by BY lanes owns an output tile of TX (x) by
+//     BY*RY (y) cells and marches through a chunk of z planes (wider tiles read
+//     fewer partial halo lines per interior line);
 //   * each lane keeps, for its RY cells, a register queue of the 2R+1 planes
 //     k-R .. k+R of v (the z neighbours never touch LDS);
 //   * the current plane k of the tile plus its R-wide xy halo is staged in LDS
@@ -107,9 +108,9 @@ __device__ __forceinline__ int xcd_remap(int b, int nb) {
   return (xcd < rem) ? xcd * (per + 1) + loc : rem * (per + 1) + (xcd - rem) * per + loc;
 }
 
-template <int R, int TY>
+template <int R, int TX, int TY>
 struct Halo {
-  static constexpr int LX = 64 + 2 * R;
+  static constexpr int LX = TX + 2 * R;
   static constexpr int LY = TY + 2 * R;
   static constexpr int N = 2 * R * LX + 2 * R * TY;
   // halo slot c -> (ly, lx) inside the LDS tile
@@ -126,7 +127,7 @@ struct Halo {
       const int side = d / (R * TY);
       const int e = d - side * R * TY;
       ly = R + e / R;
-      lx = (e % R) + side * (64 + R);
+      lx = (e % R) + side * (TX + R);
     }
   }
 };
@@ -189,14 +190,14 @@ struct Op<T, 0, true> {
   }
 };
 
-// Output tile per workgroup: 64 x BY lanes, RY rows per lane (64 x BY*RY cells).
+// Output tile per workgroup: 64*BX x BY lanes, RY rows per lane (64*BX x BY*RY cells).
 template <typename T, int KID>
 struct Tile {
   static constexpr int BY = (KID == 4) ? 8 : 4;
   static constexpr int RY = (KID == 4) ? 2 : 4;
 };
 
-template <typename T, typename C, int KID, int BY, int RY, bool FAST, bool ROCS, bool RING, int ND>
+template <typename T, typename C, int KID, int BY, int RY, bool FAST, bool ROCS, bool RING, int ND, int BX = 1>
 struct SweepTile;
 
 // MINW > 0 asks the compiler for at least MINW waves per SIMD (caps VGPRs at 512/MINW).
@@ -204,10 +205,11 @@ struct SweepTile;
 // fetched in the same step as the interior of that plane (R+1 planes ahead), when
 // the neighbouring tiles fetch it too and it is still in L2; otherwise the LDS is
 // double buffered and the halo is fetched one plane ahead.
-template <typename T, typename C, int KID, int BY, int RY, bool FAST, bool ROCS, int MINW, bool RING, int ND = kFastND>
-__global__ __launch_bounds__(64 * BY) __attribute__((amdgpu_waves_per_eu(MINW > 0 ? MINW : 1)))
+template <typename T, typename C, int KID, int BY, int RY, bool FAST, bool ROCS, int MINW, bool RING, int ND = kFastND,
+          int BX = 1>
+__global__ __launch_bounds__(64 * BX * BY) __attribute__((amdgpu_waves_per_eu(MINW > 0 ? MINW : 1)))
 void sweep_kernel(const SweepArgs<T, C> a) {
-  using ST = SweepTile<T, C, KID, BY, RY, FAST, ROCS, RING, ND>;
+  using ST = SweepTile<T, C, KID, BY, RY, FAST, ROCS, RING, ND, BX>;
   __shared__ typename ST::Lds lds;
   const int nb = a.tiles_x * a.tiles_y * a.nzc;
   const int lb = a.xcd_order ? xcd_remap(blockIdx.x, nb) : (int)blockIdx.x;
@@ -219,30 +221,30 @@ void sweep_kernel(const SweepArgs<T, C> a) {
 
 // One tile of one sweep: the z-streaming body shared by the single-grid kernel
 // and the batched persistent kernel.
-template <typename T, typename C, int KID, int BY, int RY, bool FAST, bool ROCS, bool RING, int ND>
+template <typename T, typename C, int KID, int BY, int RY, bool FAST, bool ROCS, bool RING, int ND, int BX>
 struct SweepTile {
   static constexpr int R = kradius(KID);
-  static constexpr int TY = BY * RY;
-  static constexpr int LX = Halo<R, TY>::LX, LY = Halo<R, TY>::LY;
+  static constexpr int TX = 64 * BX, TY = BY * RY;
+  static constexpr int LX = Halo<R, TX, TY>::LX, LY = Halo<R, TX, TY>::LY;
   static constexpr int S = RING ? R + 2 : 2;  // LDS tile planes
   typedef T Lds[S][LY][LX];
   __device__ static __forceinline__ void run(const SweepArgs<T, C> &a, int tile_x, int tile_y, int zc, Lds &lds);
 };
 
-template <typename T, typename C, int KID, int BY, int RY, bool FAST, bool ROCS, bool RING, int ND>
-__device__ __forceinline__ void SweepTile<T, C, KID, BY, RY, FAST, ROCS, RING, ND>::run(const SweepArgs<T, C> &a,
+template <typename T, typename C, int KID, int BY, int RY, bool FAST, bool ROCS, bool RING, int ND, int BX>
+__device__ __forceinline__ void SweepTile<T, C, KID, BY, RY, FAST, ROCS, RING, ND, BX>::run(const SweepArgs<T, C> &a,
                                                                                    int tile_x, int tile_y, int zc,
                                                                                    Lds &lds) {
   constexpr int Q = 2 * R + 1;
-  constexpr int NT = 64 * BY;
+  constexpr int NT = TX * BY;
   constexpr int NC = kncoef(KID);
-  using H = Halo<R, TY>;
+  using H = Halo<R, TX, TY>;
   constexpr int HS = (H::N + NT - 1) / NT;  // halo slots per lane
 
   const int tx = threadIdx.x, ty = threadIdx.y;
-  const int tid = ty * 64 + tx;
+  const int tid = ty * TX + tx;
   const Geom &g = a.g;
-  const int x0 = g.xb + tile_x * 64;
+  const int x0 = g.xb + tile_x * TX;
   const int y0 = g.yb + tile_y * TY;
   const int z0 = g.zb + zc * a.zchunk;
   const int z1 = min(z0 + a.zchunk, g.ze);

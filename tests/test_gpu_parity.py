@@ -110,6 +110,51 @@ def test_random_ragged_bitexact(kernel, prec):
         np.testing.assert_array_equal(U2[:, :, :nnx], v)
 
 
+@pytest.mark.parametrize("kernel", [0, 1, 5])
+@pytest.mark.parametrize("prec", ["dp", "sp"])
+def test_every_tile_variant_bitexact(kernel, prec):
+    """Every launch configuration the autotuner may pick (tile variant, XCD order, z
+    chunk) gives the oracle's result bit for bit on a ragged grid."""
+    dt = np.float64 if prec == "dp" else np.float32
+    dtype = gpu.GR_F64 if prec == "dp" else gpu.GR_F32
+    nx, ny, nz = (290, 37, 13) if orc.RADIUS[kernel] == 4 else (300, 45, 11)
+    u, v, roc2, coef = _random_problem(kernel, nx, ny, nz, dt, 900 + kernel)
+    uo, vo = u.copy(), v.copy()
+    orc.run(kernel, coef, uo, vo, roc2, nt=2)
+    orc.sweep(kernel, coef, uo, vo, roc2)
+    with gpu.Stepper(kernel, nx, ny, nz, dtype=dtype, coef=_consts(coef, kernel)) as st:
+        nvar = 16 if kernel in (0, 1, 5) else 1
+        for var in range(nvar):
+            for order, zc in ((0, 0), (1, 5)):
+                print("variant", kernel, prec, var, order, zc, flush=True)
+                st.upload(_pad(u), _pad(v), _pad(roc2) if roc2 is not None else None, _pad_coef(coef, kernel))
+                st.set_config(var, zc, 1)
+                st.set_xcd_order(order)
+                st.step(3)
+                U1, U2 = st.download(_pad(u).shape)
+                nnx = u.shape[2]
+                np.testing.assert_array_equal(U1[:, :, :nnx], uo, err_msg="variant %d" % var)
+                np.testing.assert_array_equal(U2[:, :, :nnx], vo, err_msg="variant %d" % var)
+
+
+def test_every_tile_variant_fast_identical():
+    """GR_FAST arithmetic is per cell: every tile variant gives the same bits."""
+    nx, ny, nz = 290, 37, 13
+    u, v, _, coef = _random_problem(0, nx, ny, nz, np.float32, 77)
+    ref = None
+    with gpu.Stepper(0, nx, ny, nz, dtype=gpu.GR_F32, flags=gpu.GR_FAST | gpu.GR_ROC2_SCALAR, roc2=0.13,
+                     coef=list(coef[:5])) as st:
+        for var in range(16):
+            st.upload(_pad(u), _pad(v))
+            st.set_config(var, 7 if var % 2 else 0, 1)
+            st.step(4)
+            out = st.download(_pad(u).shape)
+            if ref is None:
+                ref = out
+            np.testing.assert_array_equal(out[0], ref[0], err_msg="variant %d" % var)
+            np.testing.assert_array_equal(out[1], ref[1], err_msg="variant %d" % var)
+
+
 @pytest.mark.parametrize("prec", ["dp", "sp"])
 def test_device_closed_form_init_matches_girih(prec):
     """gr_init_fields(mode 0) reproduces girih's closed-form state bit for bit."""
