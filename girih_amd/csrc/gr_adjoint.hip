@@ -206,10 +206,11 @@ extern "C" int gr_shot_create(gr_handle *h, int rec_k, int nt, int ckpt, double 
   s->nck = (nt + s->ckpt - 1) / s->ckpt;
   s->dt_over_h = dt_over_h;
   const size_t fbytes = (size_t)P.field_elems * gr::elem_size(P.dtype);
+  hipStream_t st = gr_internal_stream(h);
   auto field = [&](void **p) -> hipError_t {
     hipError_t e = hipMalloc(p, fbytes);
-    if (e == hipSuccess) e = hipMemset(*p, 0, fbytes);  // girih's zero halo, never written
     if (e == hipSuccess) s->allocs.push_back(*p);
+    if (e == hipSuccess) e = hipMemsetAsync(*p, 0, fbytes, st);  // girih's zero halo, never written
     return e;
   };
   hipError_t e = hipSuccess;
@@ -218,6 +219,7 @@ extern "C" int gr_shot_create(gr_handle *h, int rec_k, int nt, int ckpt, double 
     if ((e = field(&p)) == hipSuccess) s->ck.push_back(p);
   }
   for (int i = 0; i < 2 && e == hipSuccess; ++i) e = field(&s->x[i]);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
   if (e != hipSuccess) {
     gr_shot_destroy(s);
     return gr::set_errorf(GR_ENOMEM, "gr_shot_create: device allocation failed: %s", hipGetErrorString(e));
@@ -296,8 +298,8 @@ extern "C" int gr_shot_adjoint(gr_shot *s, const void *res_dev, void *grad_dev) 
     const size_t before = s->allocs.size();
     auto field = [&](void **p) -> hipError_t {
       hipError_t r = hipMalloc(p, fbytes);
-      if (r == hipSuccess) r = hipMemset(*p, 0, fbytes);  // zero halo, never written
       if (r == hipSuccess) s->allocs.push_back(*p);
+      if (r == hipSuccess) r = hipMemsetAsync(*p, 0, fbytes, st);  // zero halo, ordered before first use
       return r;
     };
     for (int i = 0; i < s->ckpt + 2 && e == hipSuccess; ++i) {

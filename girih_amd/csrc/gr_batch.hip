@@ -69,10 +69,11 @@ __device__ __forceinline__ X uniform_load(const X *p) {
   return x;
 }
 
-template <typename C, int BY, int RY, bool RING>
-__global__ __launch_bounds__(64 * BY) void batch_kernel(const BatchDesc *__restrict__ d, int nprob, int nitems,
-                                                        unsigned long long *ctr, unsigned long long base, int step) {
-  using ST = SweepTile<float, C, 0, BY, RY, true, false, RING, kFastND>;
+template <typename C, int BX, int BY, int RY, bool RING>
+__global__ __launch_bounds__(64 * BX * BY) void batch_kernel(const BatchDesc *__restrict__ d, int nprob, int nitems,
+                                                             unsigned long long *ctr, unsigned long long base,
+                                                             int step) {
+  using ST = SweepTile<float, C, 0, BY, RY, true, false, RING, kFastND, BX>;
   __shared__ typename ST::Lds lds;
   __shared__ int sclaim[2];
   const bool leader = threadIdx.x == 0 && threadIdx.y == 0;
@@ -122,15 +123,17 @@ __global__ __launch_bounds__(64 * BY) void batch_kernel(const BatchDesc *__restr
 typedef void (*batch_fn)(const BatchDesc *, int, int, unsigned long long *, unsigned long long, int);
 
 struct BatchVariant {
-  int by, ry;
+  int bx, by, ry;
   bool ring;
   batch_fn f32, bf16;
 };
 
-#define GR_BV(BY, RY, RING) \
-  { BY, RY, RING, &batch_kernel<float, BY, RY, RING>, &batch_kernel<__hip_bfloat16, BY, RY, RING> }
-const BatchVariant kVariants[] = {GR_BV(8, 4, false), GR_BV(8, 4, true), GR_BV(4, 4, false), GR_BV(8, 2, true),
-                                  GR_BV(4, 4, true)};
+// tiles of 64*BX x BY*RY cells, 64*BX x BY lanes
+#define GR_BV(BX, BY, RY, RING) \
+  { BX, BY, RY, RING, &batch_kernel<float, BX, BY, RY, RING>, &batch_kernel<__hip_bfloat16, BX, BY, RY, RING> }
+const BatchVariant kVariants[] = {GR_BV(1, 8, 4, false), GR_BV(1, 8, 4, true), GR_BV(1, 4, 4, false),
+                                  GR_BV(1, 8, 2, true),  GR_BV(1, 4, 4, true), GR_BV(2, 8, 2, true),
+                                  GR_BV(2, 4, 4, true)};
 #undef GR_BV
 constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
 
@@ -162,8 +165,8 @@ struct gr_batch {
 
 namespace {
 
-int items_of(const Problem &P, int zchunk, int ty_cells, int *tx, int *ty, int *nzc, int *zc) {
-  *tx = (P.nx + 63) / 64;
+int items_of(const Problem &P, int zchunk, int tx_cells, int ty_cells, int *tx, int *ty, int *nzc, int *zc) {
+  *tx = (P.nx + tx_cells - 1) / tx_cells;
   *ty = (P.ny + ty_cells - 1) / ty_cells;
   *zc = (zchunk <= 0 || zchunk >= P.nz) ? P.nz : zchunk;
   *nzc = (P.nz + *zc - 1) / *zc;
@@ -188,7 +191,7 @@ int build_tables(gr_batch *b, bool spares) {
     D.xe = P.nx + 4;
     D.ye = P.ny + 4;
     D.ze = P.nz + 4;
-    const int cnt = items_of(P, b->zchunk, V.by * V.ry, &D.tiles_x, &D.tiles_y, &D.nzc, &D.zchunk);
+    const int cnt = items_of(P, b->zchunk, 64 * V.bx, V.by * V.ry, &D.tiles_x, &D.tiles_y, &D.nzc, &D.zchunk);
     D.item0 = item;
     item += cnt;
     D.roc2 = P.roc2;
@@ -237,7 +240,7 @@ hipError_t enqueue_steps(gr_batch *b, int table, int napply, hipStream_t st) {
   unsigned long long base = 0;
   for (int s = 0; s < napply; ++s) {
     const BatchDesc *tab = b->ddesc + (size_t)(table + (s & 1)) * n;
-    hipLaunchKernelGGL(f, dim3(b->grid), dim3(64, V.by), 0, st, tab, n, b->nitems, b->ctr, base, s);
+    hipLaunchKernelGGL(f, dim3(b->grid), dim3(64 * V.bx, V.by), 0, st, tab, n, b->nitems, b->ctr, base, s);
     base += (unsigned long long)(b->nitems + b->grid);
   }
   return hipGetLastError();
@@ -354,7 +357,7 @@ extern "C" int gr_batch_create(const gr_desc *descs, int n, gr_batch **out) {
     for (int v = 0; v < kNumVariants && e == hipSuccess; ++v) {
       const BatchVariant &V = kVariants[v];
       e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
-          &b->occupancy[v], reinterpret_cast<const void *>(b->bf16 ? V.bf16 : V.f32), 64 * V.by, 0);
+          &b->occupancy[v], reinterpret_cast<const void *>(b->bf16 ? V.bf16 : V.f32), 64 * V.bx * V.by, 0);
     }
     if (e != hipSuccess) rc = set_errorf(GR_EHIP, "gr_batch_create: %s", hipGetErrorString(e));
   }
@@ -441,7 +444,7 @@ extern "C" int gr_batch_get_stats(gr_batch *b, gr_batch_stats *out) {
   s.grid = b->grid;
   s.variant = b->variant;
   s.zchunk = b->zchunk;
-  s.tile_x = 64;
+  s.tile_x = 64 * kVariants[b->variant].bx;
   s.tile_y = kVariants[b->variant].by * kVariants[b->variant].ry;
   s.nvariants = kNumVariants;
   *out = s;

@@ -320,23 +320,30 @@ extern "C" int gr_create(const gr_desc *d, gr_handle **out) {
   hipError_t e;
   int rc = GR_OK;
   if ((e = hipGetDevice(&h->device)) != hipSuccess) rc = fail(GR_EHIP, "hipGetDevice: %s", hipGetErrorString(e));
+  // Zero fills go on the handle's own (non-blocking) stream, which does not order with
+  // the null stream: a null-stream hipMemset could still be running when the first
+  // kernel on the handle's stream writes the same memory.
+  if (rc == GR_OK && (e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking)) != hipSuccess)
+    rc = fail(GR_EHIP, "hipStreamCreate: %s", hipGetErrorString(e));
+  if (rc == GR_OK && ((e = hipEventCreate(&h->ev0)) != hipSuccess || (e = hipEventCreate(&h->ev1)) != hipSuccess))
+    rc = fail(GR_EHIP, "hipEventCreate: %s", hipGetErrorString(e));
   auto alloc = [&](void **p, size_t n) {
     if (rc != GR_OK) return;
     if ((e = hipMalloc(p, n)) != hipSuccess) {
       rc = fail(GR_ENOMEM, "hipMalloc(%zu) failed: %s", n, hipGetErrorString(e));
       return;
     }
-    if ((e = hipMemset(*p, 0, n)) != hipSuccess) rc = fail(GR_EHIP, "hipMemset: %s", hipGetErrorString(e));
+    if ((e = hipMemsetAsync(*p, 0, n, h->stream)) != hipSuccess)
+      rc = fail(GR_EHIP, "hipMemsetAsync: %s", hipGetErrorString(e));
   };
   alloc(&h->fbuf[0], fbytes);
   alloc(&h->fbuf[1], fbytes);
   const size_t cbytes = (P.flags & GR_COEF_BF16) ? (size_t)P.field_elems * 2 : fbytes;
   if (P.kernel == 0 && !(P.flags & GR_ROC2_SCALAR)) alloc(&h->alloc_roc2, cbytes);
   if (P.ncoef > 0) alloc(&h->alloc_coef, cbytes * P.ncoef);
-  if (rc == GR_OK && (e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking)) != hipSuccess)
-    rc = fail(GR_EHIP, "hipStreamCreate: %s", hipGetErrorString(e));
-  if (rc == GR_OK && ((e = hipEventCreate(&h->ev0)) != hipSuccess || (e = hipEventCreate(&h->ev1)) != hipSuccess))
-    rc = fail(GR_EHIP, "hipEventCreate: %s", hipGetErrorString(e));
+  // the fields are zero before any other stream (a batch, torch) can touch them
+  if (rc == GR_OK && (e = hipStreamSynchronize(h->stream)) != hipSuccess)
+    rc = fail(GR_EHIP, "hipStreamSynchronize: %s", hipGetErrorString(e));
   if (rc != GR_OK) {
     gr_destroy(h);
     return rc;
@@ -624,7 +631,7 @@ static int ensure_spares(gr_handle *h) {
   for (int i = 2; i < 4; ++i)
     if (!h->fbuf[i]) {
       HIP_TRY(hipMalloc(&h->fbuf[i], fbytes));
-      HIP_TRY(hipMemset(h->fbuf[i], 0, fbytes));
+      HIP_TRY(hipMemsetAsync(h->fbuf[i], 0, fbytes, h->stream));  // ordered before the spares' first use
       h->spares_synced = false;
       h->stats.device_bytes += (int64_t)fbytes;
     }
@@ -874,15 +881,22 @@ __global__ void halo_diff_kernel(const U *a, const U *b, int nx, int ny, int nz,
   if (!inner && a[o] != b[o]) *flag = 1;
 }
 
+// Coefficient m of halo-inclusive cell (i, j, k) is drawn from its logical index
+// (k*(ny+2R) + j)*(nx+2R) + i, so the values do not depend on the device layout
+// (precision, pitch) and FP32 / FP64 / bf16 problems get the same coefficients.
 template <typename C>
 __global__ void init_coef_kernel(C *coef, int ncoef, uint64_t seed, double lo0, double hi0, double lo1, double hi1,
-                                 long long field_elems) {
+                                 int nx, int R, long long pitch, long long total, long long field_elems) {
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= field_elems) return;
+  if (t >= total) return;
+  const int nnx = nx + 2 * R;
+  const long long row = t / nnx;
+  const int i = (int)(t - row * nnx);
+  const long long o = row * pitch + i;
   for (int m = 0; m < ncoef; ++m) {
     const double x = u01(seed, 1 + m, (uint64_t)t);
     const double val = m == 0 ? lo0 + (hi0 - lo0) * x : lo1 + (hi1 - lo1) * x;
-    coef[(long long)m * field_elems + t] = (C)val;
+    coef[(long long)m * field_elems + o] = (C)val;
   }
 }
 
@@ -942,16 +956,19 @@ extern "C" int gr_init_coef_uniform(gr_handle *h, uint64_t seed, double lo0, dou
   const Problem &P = h->P;
   if (P.ncoef == 0 || !h->alloc_coef) return fail(GR_EINVAL, "gr_init_coef_uniform: kernel has no coefficient arrays");
   const int bs = 256;
-  const unsigned nb = (unsigned)((P.field_elems + bs - 1) / bs);
+  const long long total = (long long)(P.nx + 2 * P.R) * P.rows * P.planes;
+  const unsigned nb = (unsigned)((total + bs - 1) / bs);
+  const long long pitch = P.pitch;
   if (P.flags & GR_COEF_BF16)
     hipLaunchKernelGGL(init_coef_kernel<__hip_bfloat16>, dim3(nb), dim3(bs), 0, h->stream,
-                       (__hip_bfloat16 *)h->alloc_coef, P.ncoef, seed, lo0, hi0, lo1, hi1, P.field_elems);
+                       (__hip_bfloat16 *)P.coef, P.ncoef, seed, lo0, hi0, lo1, hi1, P.nx, P.R, pitch, total,
+                       P.field_elems);
   else if (P.dtype == GR_F64)
-    hipLaunchKernelGGL(init_coef_kernel<double>, dim3(nb), dim3(bs), 0, h->stream, (double *)h->alloc_coef, P.ncoef,
-                       seed, lo0, hi0, lo1, hi1, P.field_elems);
+    hipLaunchKernelGGL(init_coef_kernel<double>, dim3(nb), dim3(bs), 0, h->stream, (double *)P.coef, P.ncoef, seed,
+                       lo0, hi0, lo1, hi1, P.nx, P.R, pitch, total, P.field_elems);
   else
-    hipLaunchKernelGGL(init_coef_kernel<float>, dim3(nb), dim3(bs), 0, h->stream, (float *)h->alloc_coef, P.ncoef,
-                       seed, lo0, hi0, lo1, hi1, P.field_elems);
+    hipLaunchKernelGGL(init_coef_kernel<float>, dim3(nb), dim3(bs), 0, h->stream, (float *)P.coef, P.ncoef, seed,
+                       lo0, hi0, lo1, hi1, P.nx, P.R, pitch, total, P.field_elems);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipStreamSynchronize(h->stream));
   return GR_OK;

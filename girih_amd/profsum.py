@@ -64,6 +64,7 @@ def main():
     ap.add_argument("--key")
     ap.add_argument("--skip", type=int, default=0, help="leading dispatches of the kernel to ignore")
     ap.add_argument("--steps-per-launch", type=int, default=1)
+    ap.add_argument("--launch-config", default=None, help="the launch configuration profiled (bench.py --config)")
     a = ap.parse_args()
 
     rows = trace_rows(a.trace)
@@ -127,7 +128,7 @@ def main():
                 data = json.load(fh)
         data[a.key] = {"bytes_per_launch": fb + wb, "fetch_bytes": fb, "write_bytes": wb,
                        "bytes_per_lup": (fb + wb) / a.lups, "avg_launch_us": avg_ns / 1e3,
-                       "steps_per_launch": a.steps_per_launch,
+                       "steps_per_launch": a.steps_per_launch, "launch_config": a.launch_config,
                        "summary": os.path.relpath(a.out, os.path.dirname(os.path.abspath(a.traffic_json)))}
         with open(a.traffic_json, "w") as fh:
             json.dump(data, fh, indent=1, sort_keys=True)

@@ -47,7 +47,7 @@ def _alone(size, flags, seed, i, napply, src=None, pre=0):
 
 
 @pytest.mark.parametrize("bf16", [True, False])
-@pytest.mark.parametrize("variant,zchunk", [(0, 0), (1, 8), (2, 0), (3, 16), (4, 7)])
+@pytest.mark.parametrize("variant,zchunk", [(0, 0), (1, 8), (2, 0), (3, 16), (4, 7), (5, 9), (6, 0)])
 def test_batch_equals_single_problem_steps(bf16, variant, zchunk):
     flags = gpu.GR_FAST | (gpu.GR_COEF_BF16 if bf16 else 0)
     src = np.linspace(0.5, -0.2, 7)

@@ -155,6 +155,28 @@ def test_every_tile_variant_fast_identical():
             np.testing.assert_array_equal(out[1], ref[1], err_msg="variant %d" % var)
 
 
+def test_device_random_coefficients_do_not_depend_on_precision():
+    """gr_init_coef_uniform and gr_init_fields(mode 1) draw from logical cell indices:
+    an FP32 and an FP64 problem with the same seeds hold the same values up to
+    rounding, so their runs differ only by FP32 arithmetic (and the fields a new
+    problem is created with are zero before its first kernel runs)."""
+    # 256^3: large enough that zero-filling a new problem takes longer than its first
+    # kernels (the fills must be ordered with the handle's stream, not the null stream)
+    for kernel, n in ((5, (70, 33, 21)), (3, (40, 20, 18)), (4, (36, 20, 18)), (5, (256, 256, 256))):
+        hs = {}
+        for dt in (gpu.GR_F64, gpu.GR_F32):
+            st = gpu.Stepper(kernel, *n, dtype=dt)
+            st.init_coef_uniform(5, 0.3, 0.5, 0.0, 0.1)
+            st.init_fields(1, 12345)
+            st.step(12)
+            hs[dt] = st
+        c = hs[gpu.GR_F32].compare(1, hs[gpu.GR_F64], 1)
+        print("kernel", kernel, "FP32 vs FP64 rel L2", c["rel_l2"], flush=True)
+        assert c["sq_ref"] > 0 and c["rel_l2"] < 1e-5
+        for st in hs.values():
+            st.close()
+
+
 @pytest.mark.parametrize("prec", ["dp", "sp"])
 def test_device_closed_form_init_matches_girih(prec):
     """gr_init_fields(mode 0) reproduces girih's closed-form state bit for bit."""
