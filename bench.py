@@ -340,8 +340,10 @@ def main():
     bytes_per_cell_launch = 12.0 if steps_per_launch == 1 else 16.0
     algo_bytes = bytes_per_cell_launch * cells
     achieved = algo_bytes / (launch_ms * 1e-3) / 1e9
-    tr = load_traffic("wave25_f32_%d" % n)
-    traffic = tr["bytes_per_launch"] if tr and abs(tr.get("steps_per_launch", 1) - steps_per_launch) < 1e-9 else None
+    launch_config = "%d,%d,%d,%d" % (s32["tile_variant"], s32["zchunk"], s32["xcd_order"], s32["t_block"])
+    # profiled traffic is kept per launch configuration (it depends on the tile and z chunk)
+    tr = load_traffic("wave25_f32_%d@%s" % (n, launch_config))
+    traffic = tr["bytes_per_launch"] if tr and tr.get("launch_config") == launch_config else None
 
     out = {
         "metric": "GLUP/s + % HBM roofline, 25pt wave 1024^3; FP32 vs FP64 GPU at stated error",
@@ -361,14 +363,14 @@ def main():
                                % (n, K),
                    "nx": n, "ny": n, "nz": n, "nt": K, "t_block": s32["t_block"],
                    "tile": [s32["tile_x"], s32["tile_y"]], "zchunk": s32["zchunk"],
-                   "launch_config": "%d,%d,%d,%d" % (s32["tile_variant"], s32["zchunk"], s32["xcd_order"],
-                                                     s32["t_block"]),
+                   "launch_config": launch_config,
                    "autotuned": config is None},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "algo_bytes_per_launch": algo_bytes, "avg_launch_ms": round(launch_ms, 5),
                      "steps_per_launch": steps_per_launch,
-                     "traffic_bytes_per_lup": (round(traffic / (cells * steps_per_launch), 3) if traffic else None)},
+                     "traffic_bytes_per_lup": (round(traffic / (cells * steps_per_launch), 3) if traffic else None),
+                     "traffic_profile": tr.get("summary") if traffic else None},
     }
     if not args.no_fp64:
         st64, t64, s64 = run_wave(gpu.GR_F64, 0, cfg, K, W)
