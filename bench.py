@@ -127,9 +127,10 @@ def _timed_steps(st, napply):
     return time.perf_counter() - t0
 
 
-def extra_first_order(kernel, n, nt, coef=None, fp32=True):
+def extra_first_order(kernel, n, nt, coef=None, fp32=True, graph=False):
     """girih kernels 1 / 5: FP64 (bit-exact path) and optionally FP32 on the same
-    seeded input, autotuned; FP32 error vs FP64 reduced on the device."""
+    seeded input, autotuned; FP32 error vs FP64 reduced on the device.  graph=True
+    replays the step loop from a HIP graph (captured by an untimed first run)."""
     from girih_amd import gpu
     napply = 2 * ((nt + 1) // 2)  # girih's SB/MWD application count for even nt
     out = {}
@@ -141,6 +142,10 @@ def extra_first_order(kernel, n, nt, coef=None, fp32=True):
             st.init_coef_uniform(5, 0.3, 0.5, 0.0, 0.1)
         st.init_fields(1, 12345)
         tuned = st.autotune(4)
+        if graph:
+            st.set_graph(True)
+            st.init_fields(1, 12345)
+            st.step(napply)  # capture
         st.init_fields(1, 12345)
         t = _timed_steps(st, napply)
         s = st.stats()
@@ -150,7 +155,7 @@ def extra_first_order(kernel, n, nt, coef=None, fp32=True):
                      "sweep_roofline_frac": round(gl * b_sweep / HBM_PEAK_GBS, 4), "bytes_per_lup_sweep": b_sweep,
                      "t_block": tuned["t_block"], "zchunk": tuned["zchunk"], "tile": [tuned["tile_x"],
                                                                                      tuned["tile_y"]],
-                     "launches": s["last_launches"]}
+                     "launches": s["last_launches"], "graph": graph}
         keep[name] = st
     if "f32" in keep:
         c = keep["f32"].compare(1, keep["f64"], 1)
@@ -387,7 +392,7 @@ def main():
     if not args.no_extras:
         print("bench: extras", file=sys.stderr, flush=True)
         out["extras"] = {
-            "jacobi7_f64_128": extra_first_order(1, 128, 100, coef=[0.4, 0.1], fp32=False),
+            "jacobi7_f64_128": extra_first_order(1, 128, 100, coef=[0.4, 0.1], fp32=False, graph=True),
             "var7_nosym_512": extra_first_order(5, 512, 500),
             "batch16_wave25_bf16": extra_batch(500),
             "adjoint_wave25_400": extra_adjoint(),

@@ -228,6 +228,17 @@ hipError_t sweep_fields(const Problem &P, void *u, const void *v, const void *sr
 
 using namespace gr;
 
+// What a captured step sequence depends on besides the (fixed) field allocations.
+struct GraphKey {
+  int napply, parity, lu0, lu1, variant, zchunk, t_block, xcd_order, fuse;
+  long long napplied;  // -1 unless a source is active (its pointers are baked into the graph)
+  bool operator==(const GraphKey &o) const {
+    return napply == o.napply && parity == o.parity && lu0 == o.lu0 && lu1 == o.lu1 && variant == o.variant &&
+           zchunk == o.zchunk && t_block == o.t_block && xcd_order == o.xcd_order && fuse == o.fuse &&
+           napplied == o.napplied;
+  }
+};
+
 struct gr_handle {
   Problem P;
   LaunchCfg lc;
@@ -248,7 +259,14 @@ struct gr_handle {
   long long napplied = 0;
   gr_stats stats{};
   int device = 0;
+  // graph mode (gr_set_graph): the last captured step sequence and the state it leaves
+  bool use_graph = false;
+  hipGraphExec_t gexec = nullptr;
+  GraphKey gkey{};
+  int gparity = 0, glu[2] = {0, 1}, glaunches = 0;
 };
+
+static void drop_graph(gr_handle *h);
 
 static void *field_base(const gr_handle *h, void *alloc) {
   return alloc ? (char *)alloc + (size_t)h->P.xoff * elem_size(h->P.dtype) : nullptr;
@@ -359,6 +377,7 @@ extern "C" int gr_create(const gr_desc *d, gr_handle **out) {
 extern "C" int gr_destroy(gr_handle *h) {
   if (!h) return GR_OK;
   if (h->stream) (void)hipStreamSynchronize(h->stream);
+  if (h->gexec) (void)hipGraphExecDestroy(h->gexec);
   for (void *p : {h->fbuf[0], h->fbuf[1], h->fbuf[2], h->fbuf[3], h->alloc_roc2, h->alloc_coef, h->src, h->flag})
     if (p) (void)hipFree(p);
   if (h->ev0) (void)hipEventDestroy(h->ev0);
@@ -431,6 +450,7 @@ extern "C" int gr_download(gr_handle *h, const int ld[3], void *U1, void *U2) {
 extern "C" int gr_set_source(gr_handle *h, int i, int j, int k, const double *values, int n) {
   if (!h) return fail(GR_EINVAL, "gr_set_source: null handle");
   const Problem &P = h->P;
+  drop_graph(h);  // a captured step sequence holds the old source pointers
   if (h->src) {
     HIP_TRY(hipFree(h->src));
     h->src = nullptr;
@@ -458,6 +478,15 @@ extern "C" int gr_set_source(gr_handle *h, int i, int j, int k, const double *va
 
 static int check_halo_equal(gr_handle *h);
 static int ensure_spares(gr_handle *h);
+static int enqueue_apps(gr_handle *h, int napply, bool fuse, int *nlaunch);
+static int finish_step(gr_handle *h, int napply, int launches);
+static void drop_graph(gr_handle *h) {
+  if (h->gexec) {
+    (void)hipStreamSynchronize(h->stream);  // a replay may still be running
+    (void)hipGraphExecDestroy(h->gexec);
+  }
+  h->gexec = nullptr;
+}
 
 // One application U(w) <- f(U(r)) with the handle's sweep configuration.
 static hipError_t single_app(gr_handle *h, long long app) {
@@ -508,7 +537,59 @@ extern "C" int gr_step(gr_handle *h, int napply) {
       h->spares_synced = true;
     }
   }
+  // graph mode: replay the launch sequence of an identical earlier call
+  GraphKey key{napply, h->parity, h->lu[0], h->lu[1], h->lc.variant, h->lc.zchunk, h->lc.t_block, h->lc.xcd_order,
+               fuse ? 1 : 0, -1};
+  if (h->src && h->napplied < h->nsrc) key.napplied = h->napplied;  // source pointers are baked in
+  if (h->use_graph && napply > 0) {
+    if (!(h->gexec && h->gkey == key)) {
+      drop_graph(h);
+      const int parity0 = h->parity, lu0 = h->lu[0], lu1 = h->lu[1];
+      hipGraph_t g = nullptr;
+      HIP_TRY(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+      int launches = 0;
+      const int erc = enqueue_apps(h, napply, fuse, &launches);
+      hipError_t e = hipStreamEndCapture(h->stream, &g);
+      if (erc == GR_OK && e == hipSuccess) e = hipGraphInstantiate(&h->gexec, g, nullptr, nullptr, 0);
+      if (g) (void)hipGraphDestroy(g);
+      h->gparity = h->parity;
+      h->glu[0] = h->lu[0];
+      h->glu[1] = h->lu[1];
+      h->glaunches = launches;
+      h->parity = parity0;  // the state advances when the graph runs, below
+      h->lu[0] = lu0;
+      h->lu[1] = lu1;
+      if (erc != GR_OK) {
+        drop_graph(h);
+        return erc;
+      }
+      if (e != hipSuccess) {
+        drop_graph(h);
+        return fail(GR_EHIP, "step graph capture: %s", hipGetErrorString(e));
+      }
+      h->gkey = key;
+    }
+    HIP_TRY(hipEventRecord(h->ev0, h->stream));
+    HIP_TRY(hipGraphLaunch(h->gexec, h->stream));
+    h->parity = h->gparity;
+    h->lu[0] = h->glu[0];
+    h->lu[1] = h->glu[1];
+    return finish_step(h, napply, h->glaunches);
+  }
   HIP_TRY(hipEventRecord(h->ev0, h->stream));
+  int launches = 0;
+  if ((rc = enqueue_apps(h, napply, fuse, &launches))) return rc;
+  return finish_step(h, napply, launches);
+}
+
+// Launch napply applications on the handle's stream (fused passes where enabled,
+// then single sweeps), advancing the host-side parity and buffer mapping.  Only
+// kernel launches: the sequence may be captured into a HIP graph.
+static int enqueue_apps(gr_handle *h, int napply, bool fuse, int *nlaunch) {
+  const Problem &P = h->P;
+  const size_t es = elem_size(P.dtype);
+  const int tl = h->lc.t_block;
+  const bool wave = P.kernel == 0;
   int s = 0, launches = 0;
   if (fuse) {
     // kernel 0 keeps (u^(n-1), u^n) exact after every pass; first-order kernels
@@ -545,6 +626,12 @@ extern "C" int gr_step(gr_handle *h, int napply) {
     hipError_t e = single_app(h, h->napplied + s);
     if (e != hipSuccess) return fail(GR_EHIP, "sweep launch failed: %s", hipGetErrorString(e));
   }
+  *nlaunch = launches;
+  return GR_OK;
+}
+
+static int finish_step(gr_handle *h, int napply, int launches) {
+  const Problem &P = h->P;
   HIP_TRY(hipEventRecord(h->ev1, h->stream));
   HIP_TRY(hipEventSynchronize(h->ev1));
   float ms = 0.f;
@@ -553,6 +640,13 @@ extern "C" int gr_step(gr_handle *h, int napply) {
   h->stats.last_ms = ms;
   h->stats.last_updates = (int64_t)napply * P.nx * P.ny * P.nz;
   h->stats.last_launches = launches;
+  return GR_OK;
+}
+
+extern "C" int gr_set_graph(gr_handle *h, int on) {
+  if (!h) return fail(GR_EINVAL, "gr_set_graph: null handle");
+  h->use_graph = on != 0;
+  if (!h->use_graph) drop_graph(h);
   return GR_OK;
 }
 
@@ -630,6 +724,7 @@ static int ensure_spares(gr_handle *h) {
   const size_t fbytes = (size_t)h->P.field_elems * elem_size(h->P.dtype);
   for (int i = 2; i < 4; ++i)
     if (!h->fbuf[i]) {
+      drop_graph(h);
       HIP_TRY(hipMalloc(&h->fbuf[i], fbytes));
       HIP_TRY(hipMemsetAsync(h->fbuf[i], 0, fbytes, h->stream));  // ordered before the spares' first use
       h->spares_synced = false;

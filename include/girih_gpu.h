@@ -153,6 +153,11 @@ int gr_init_coef_uniform(gr_handle *h, uint64_t seed, double lo0, double hi0, do
 int gr_set_config(gr_handle *h, int tile_variant, int zchunk, int t_block);
 /* Workgroup order of the sweep kernels: 1 = XCD-contiguous tile ranges, 0 = plain. */
 int gr_set_xcd_order(gr_handle *h, int xcd_order);
+/* Graph mode (no counterpart in girih): gr_step captures its launch sequence into a
+ * HIP graph once and replays it while the step count, parity, configuration and
+ * source window repeat (one launch per call instead of one per sweep; for small,
+ * launch-bound grids).  Results are identical to eager launches. */
+int gr_set_graph(gr_handle *h, int on);
 /* Autotuner (replaces girih's CPU tile/thread-group search, diamond_utils.c:691-847):
  * times every tile variant x z-chunk x fused depth candidate for `steps` steps on
  * scratch copies of the fields, keeps the fastest, and reports it in *best (may be
