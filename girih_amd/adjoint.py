@@ -22,7 +22,8 @@ from . import gpu, physics
 class Shot:
     """A kernel-0 problem with a point source and a receiver plane (gr_shot)."""
 
-    def __init__(self, nx, ny, nz, nt, dt_over_h, src_point, src, rec_k, ckpt=50, dtype=gpu.GR_F32, coef=None):
+    def __init__(self, nx, ny, nz, nt, dt_over_h, src_point, src, rec_k, ckpt=50, dtype=gpu.GR_F32, coef=None,
+                 tune=True):
         flags = gpu.GR_FAST if dtype == gpu.GR_F32 else 0
         self.nx, self.ny, self.nz, self.nt = nx, ny, nz, nt
         self.dt_over_h = float(dt_over_h)
@@ -30,8 +31,12 @@ class Shot:
         self.torch_dtype = torch.float32 if dtype == gpu.GR_F32 else torch.float64
         self.stepper = gpu.Stepper(0, nx, ny, nz, dtype=dtype, flags=flags,
                                    coef=physics.fd8_coef() if coef is None else coef)
-        self.stepper.set_config(zchunk=64)
         self.stepper.set_source(*src_point, src)
+        if tune:  # tile shape and z chunk of the sweeps (times scratch fields; results do not depend on it)
+            self.stepper.autotune(4)
+            self.stepper.set_config(t_block=1)  # shots step one application at a time
+        else:
+            self.stepper.set_config(zchunk=64)
         self._s = ctypes.c_void_p()
         gpu.check(gpu.lib().gr_shot_create(self.stepper._h, int(rec_k), int(nt), int(ckpt), self.dt_over_h,
                                            ctypes.byref(self._s)))

@@ -14,12 +14,13 @@
 // so the forward sweep kernel steps it unchanged.  With r L x_{a+1} =
 // x_{a+2} - 2x_{a+1} + x_a - s_a e_src and dr/dm = -r^2 h^2/dt^2:
 //   dJ/dm = -(h/dt)^2 * sum_a nu_{a+2} * ((x_{a+2} - x_{a+1}) - (x_{a+1} - x_a) - s_a e_src),
-// a pointwise accumulation (FP64) with no second stencil pass.
+// a pointwise accumulation (differences in FP64, sum in the field type) with no
+// second stencil pass.
 //
 // The forward pass stores (x_{cK}, x_{cK+1}) every K applications; the backward
 // pass recomputes each segment's K states from its checkpoint into a buffer and
 // walks it backwards.  Memory: 2*ceil(nt/K) + 2 fields for the forward pass;
-// the first backward pass adds K + 4 fields and an FP64 accumulator.
+// the first backward pass adds K + 4 fields and a dense accumulator.
 #include <hip/hip_runtime.h>
 #include <hip/hip_bf16.h>
 
@@ -58,28 +59,33 @@ __global__ void inject_kernel_same(T *nu, const T *roc2, long long sy, long long
   nu[o] = nu[o] + roc2[o] * res[(long long)j * nx + i];
 }
 
-// g += nu * ((x2 - x1) - (x1 - x0) - s e_src), in FP64 over the interior (g dense [nz][ny][nx])
+// g += nu * ((x2 - x1) - (x1 - x0) - s e_src) over the interior (g dense [nz][ny][nx]).
+// The differences are formed in FP64 (exact for FP32 states); g has the field type:
+// an FP32 shot's gradient error is set by its FP32 traces (~1e-6 relative), far
+// above FP32 accumulation over the steps.
 template <typename T>
-__global__ void grad_accum_kernel(double *g, const T *nu, const T *x2, const T *x1, const T *x0, long long sy,
-                                  long long sz, int nx, int ny, int R, const T *src, long long src_idx) {
+__global__ void grad_accum_kernel(T *g, const T *nu, const T *x2, const T *x1, const T *x0, long long sy, long long sz,
+                                  int nx, int ny, int R, const T *src, long long src_idx) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y, k = blockIdx.z;
   if (i >= nx) return;
   const long long o = (long long)(k + R) * sz + (long long)(j + R) * sy + (i + R);
   const double a = (double)x2[o], b = (double)x1[o], c = (double)x0[o];
   double d = (a - b) - (b - c);
   if (src && o == src_idx) d = d - (double)*src;
-  g[((long long)k * ny + j) * nx + i] += (double)nu[o] * d;
+  T &gc = g[((long long)k * ny + j) * nx + i];
+  gc = (T)((double)gc + (double)nu[o] * d);
 }
 
 template <typename T>
-__global__ void grad_final_kernel(const double *g, T *out, long long n, double scale) {
+__global__ void grad_final_kernel(const T *g, T *out, long long n, double scale) {
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < n) out[t] = (T)(scale * g[t]);
+  if (t < n) out[t] = (T)(scale * (double)g[t]);
 }
 
-__global__ void zero_kernel(double *g, long long n) {
+template <typename T>
+__global__ void zero_kernel(T *g, long long n) {
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < n) g[t] = 0.0;
+  if (t < n) g[t] = T(0);
 }
 
 // roc2 = dt_over_h^2 / m over the whole field (halo cells take the nearest interior m)
@@ -112,7 +118,7 @@ struct gr_shot {
   std::vector<void *> seg;     // ckpt+2 recompute fields
   void *nu[2] = {nullptr, nullptr};
   void *x[2] = {nullptr, nullptr};  // forward ping-pong
-  double *g = nullptr;
+  void *g = nullptr;  // gradient accumulator, field type, dense interior
   bool have_forward = false;
 };
 
@@ -151,16 +157,18 @@ hipError_t inject_any(const Problem &P, void *nu, int rec_k, const void *res, hi
   return hipGetLastError();
 }
 
-hipError_t grad_any(const Problem &P, double *g, const void *nu, const void *x2, const void *x1, const void *x0,
+hipError_t grad_any(const Problem &P, void *g, const void *nu, const void *x2, const void *x1, const void *x0,
                     const void *src, long long sidx, hipStream_t st) {
   const long long sz = (long long)P.pitch * P.rows;
   const dim3 grid((P.nx + 255) / 256, P.ny, P.nz);
   if (P.dtype == GR_F64)
-    hipLaunchKernelGGL(grad_accum_kernel<double>, grid, dim3(256), 0, st, g, (const double *)nu, (const double *)x2,
+    hipLaunchKernelGGL(grad_accum_kernel<double>, grid, dim3(256), 0, st, (double *)g, (const double *)nu,
+                       (const double *)x2,
                        (const double *)x1, (const double *)x0, (long long)P.pitch, sz, P.nx, P.ny, P.R,
                        (const double *)src, sidx);
   else
-    hipLaunchKernelGGL(grad_accum_kernel<float>, grid, dim3(256), 0, st, g, (const float *)nu, (const float *)x2,
+    hipLaunchKernelGGL(grad_accum_kernel<float>, grid, dim3(256), 0, st, (float *)g, (const float *)nu,
+                       (const float *)x2,
                        (const float *)x1, (const float *)x0, (long long)P.pitch, sz, P.nx, P.ny, P.R,
                        (const float *)src, sidx);
   return hipGetLastError();
@@ -308,7 +316,7 @@ extern "C" int gr_shot_adjoint(gr_shot *s, const void *res_dev, void *grad_dev) 
     }
     for (int i = 0; i < 2 && e == hipSuccess; ++i) e = field(&s->nu[i]);
     if (e == hipSuccess) {
-      e = hipMalloc(&s->g, sizeof(double) * (size_t)ncell);
+      e = hipMalloc(&s->g, gr::elem_size(P.dtype) * (size_t)ncell);
       if (e == hipSuccess) s->allocs.push_back(s->g);
     }
     if (e != hipSuccess) {  // release the partial set so a later call starts clean
@@ -322,7 +330,12 @@ extern "C" int gr_shot_adjoint(gr_shot *s, const void *res_dev, void *grad_dev) 
   }
   for (int i = 0; i < 2 && e == hipSuccess; ++i) e = hipMemsetAsync(s->nu[i], 0, fbytes, st);
   if (e == hipSuccess) {
-    hipLaunchKernelGGL(zero_kernel, dim3((unsigned)((ncell + 255) / 256)), dim3(256), 0, st, s->g, ncell);
+    if (P.dtype == GR_F64)
+      hipLaunchKernelGGL(zero_kernel<double>, dim3((unsigned)((ncell + 255) / 256)), dim3(256), 0, st, (double *)s->g,
+                         ncell);
+    else
+      hipLaunchKernelGGL(zero_kernel<float>, dim3((unsigned)((ncell + 255) / 256)), dim3(256), 0, st, (float *)s->g,
+                         ncell);
     e = hipGetLastError();
   }
   void *n0 = s->nu[0], *n1 = s->nu[1];  // nu_{a+4}, nu_{a+3}
@@ -354,9 +367,11 @@ extern "C" int gr_shot_adjoint(gr_shot *s, const void *res_dev, void *grad_dev) 
     const double scale = -1.0 / (s->dt_over_h * s->dt_over_h);
     const unsigned nb = (unsigned)((ncell + 255) / 256);
     if (P.dtype == GR_F64)
-      hipLaunchKernelGGL(grad_final_kernel<double>, dim3(nb), dim3(256), 0, st, s->g, (double *)grad_dev, ncell, scale);
+      hipLaunchKernelGGL(grad_final_kernel<double>, dim3(nb), dim3(256), 0, st, (const double *)s->g,
+                         (double *)grad_dev, ncell, scale);
     else
-      hipLaunchKernelGGL(grad_final_kernel<float>, dim3(nb), dim3(256), 0, st, s->g, (float *)grad_dev, ncell, scale);
+      hipLaunchKernelGGL(grad_final_kernel<float>, dim3(nb), dim3(256), 0, st, (const float *)s->g, (float *)grad_dev,
+                         ncell, scale);
     e = hipGetLastError();
   }
   if (e == hipSuccess) e = hipStreamSynchronize(st);

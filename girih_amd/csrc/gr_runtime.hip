@@ -750,6 +750,9 @@ extern "C" int gr_autotune(gr_handle *h, int steps, gr_stats *best) {
   std::vector<int> zchunks = {0};
   for (int zc : {512, 256, 128, 64, 32})
     if (zc < nz) zchunks.push_back(zc);
+  if (nz <= 256)  // small grids need short chunks to fill 256 CUs
+    for (int zc : {16, 8})
+      if (zc < nz) zchunks.push_back(zc);
   LaunchCfg best_lc = h->lc;
   float best_ms = 1e30f;
   auto trial = [&](const LaunchCfg &lc) -> int {
