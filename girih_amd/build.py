@@ -21,7 +21,8 @@ DRIVER_DIR = os.path.join(PKG, "driver")
 DRIVER_BINS = {"dp": os.path.join(DRIVER_DIR, "bin", "mwd_kernel_gpu_dp"),
                "sp": os.path.join(DRIVER_DIR, "bin", "mwd_kernel_gpu_sp")}
 
-HIP_SOURCES = ["gr_runtime.hip", "gr_tblock.hip", "gr_model.hip", "gr_batch.hip", "gr_adjoint.hip"]
+HIP_SOURCES = ["gr_launch_f32.hip", "gr_launch_f64.hip", "gr_launch_bf16.hip", "gr_runtime.hip", "gr_tblock.hip",
+               "gr_model.hip", "gr_batch.hip", "gr_adjoint.hip"]
 ARCH = os.environ.get("GIRIH_OFFLOAD_ARCH", "gfx950")
 
 # -ffp-contract=off: the reference-order kernels must not fuse multiply-adds
@@ -45,19 +46,37 @@ def run(cmd):
 
 
 def build_lib(verbose=False):
+    """Compile every HIP source (in parallel, MAX_JOBS or up to 8 at once) and link the library."""
     os.makedirs(LIBDIR, exist_ok=True)
-    objs = []
+    jobs = []
     for src in HIP_SOURCES:
         path = os.path.join(CSRC, src)
         if not os.path.exists(path):
-            continue
+            raise FileNotFoundError(path)
         obj = os.path.join(LIBDIR, os.path.splitext(src)[0] + ".o")
-        run([hipcc()] + HIPCC_FLAGS + ["-I" + os.path.join(REPO, "include"), "-c", path, "-o", obj])
+        jobs.append(([hipcc()] + HIPCC_FLAGS + ["-I" + os.path.join(REPO, "include"), "-c", path, "-o", obj], obj))
+    width = max(1, min(8, int(os.environ.get("MAX_JOBS", os.cpu_count() or 1))))
+    running, objs = [], []
+    for cmd, obj in jobs:
+        print("+", " ".join(cmd), flush=True)
+        running.append((subprocess.Popen(cmd), cmd))
         objs.append(obj)
+        while len(running) >= width:
+            _wait_first(running)
+    while running:
+        _wait_first(running)
     tmp = LIB + ".tmp"
     run([hipcc(), "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs)
     os.replace(tmp, LIB)
     return LIB
+
+
+def _wait_first(running):
+    proc, cmd = running.pop(0)
+    if proc.wait() != 0:
+        for p, _ in running:
+            p.wait()
+        raise subprocess.CalledProcessError(proc.returncode, cmd)
 
 
 def build_driver():

@@ -73,7 +73,7 @@ template <typename C, int BX, int BY, int RY, bool RING>
 __global__ __launch_bounds__(64 * BX * BY) void batch_kernel(const BatchDesc *__restrict__ d, int nprob, int nitems,
                                                              unsigned long long *ctr, unsigned long long base,
                                                              int step) {
-  using ST = SweepTile<float, C, 0, BY, RY, true, false, RING, kFastND, BX>;
+  using ST = SweepTile<float, C, 0, BY, RY, true, false, RING, kFastND, BX, true>;  // a step streams every problem
   __shared__ typename ST::Lds lds;
   __shared__ int sclaim[2];
   const bool leader = threadIdx.x == 0 && threadIdx.y == 0;

@@ -30,6 +30,9 @@ int num_tile_variants(int kid, int dtype);
 void tile_of_variant(int kid, int dtype, int variant, int *tx, int *ty);
 
 size_t elem_size(int dtype);
+// Whether a sweep of P should use non-temporal hints: its fields and coefficient
+// arrays (the data one step streams) exceed what the caches keep between steps.
+int streaming_sweep(const Problem &P);
 int set_error(int code, const char *msg);
 int set_errorf(int code, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
 // One sweep u <- f(v); kernel 0 reads u^(n-1) from uold when given (else from u).

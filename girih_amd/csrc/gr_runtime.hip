@@ -22,7 +22,7 @@
 
 #include "../../include/girih_gpu.h"
 #include "gr_internal.hpp"
-#include "gr_sweep.hpp"
+#include "gr_launch.hpp"
 
 namespace {
 
@@ -60,23 +60,13 @@ int set_errorf(int code, const char *fmt, ...) {
 
 size_t elem_size(int dtype) { return dtype == GR_F64 ? 8 : 4; }
 
-// ------------------------------------------------------------------ launch plumbing
-template <typename T, typename C, int KID, bool FAST, bool ROCS, int BY, int RY, int MINW, bool RING = false, int BX = 1>
-static hipError_t launch_cfg(const SweepArgs<T, C> &a0, const LaunchCfg &lc, hipStream_t st) {
-  SweepArgs<T, C> a = a0;
-  const int TX = 64 * BX, TY = BY * RY;
-  const int bx = a.g.xe - a.g.xb, by = a.g.ye - a.g.yb, bz = a.g.ze - a.g.zb;
-  a.tiles_x = (bx + TX - 1) / TX;
-  a.tiles_y = (by + TY - 1) / TY;
-  a.zchunk = lc.zchunk > 0 ? std::min(lc.zchunk, bz) : bz;
-  a.nzc = (bz + a.zchunk - 1) / a.zchunk;
-  a.xcd_order = lc.xcd_order;
-  const int nb = a.tiles_x * a.tiles_y * a.nzc;
-  hipLaunchKernelGGL((sweep_kernel<T, C, KID, BY, RY, FAST, ROCS, MINW, RING, kFastND, BX>), dim3(nb), dim3(TX, BY), 0,
-                     st, a);
-  return hipGetLastError();
+int streaming_sweep(const Problem &P) {
+  const double cbytes = (P.flags & GR_COEF_BF16) ? 2.0 : (double)elem_size(P.dtype);
+  const double words = 2.0 * elem_size(P.dtype) + cbytes * (P.ncoef + (P.roc2 ? 1 : 0));
+  return (double)P.field_elems * words > 512.0 * 1024 * 1024 ? 1 : 0;  // 2x the 256 MB Infinity Cache
 }
 
+// ------------------------------------------------------------------ tile variants
 // Tile variants selectable at run time for the kernels on the hot path (the
 // other kernels use their default tile).  Lanes are 64*BX x BY, RY rows per lane:
 //   0: default (64 x 16)                 1: 64x32 (8 x 4)   2: 64x16 (8 x 2)
@@ -100,63 +90,6 @@ void tile_of_variant(int kid, int dtype, int variant, int *tx, int *ty) {
   const bool ok = variant > 0 && variant < num_tile_variants(kid, dtype);
   *tx = ok ? cols[variant] : 64;
   *ty = ok ? rows[variant] : 16;
-}
-
-// The LDS plane ring holds R+2 planes of the tile plus halo; where that exceeds the
-// 160 KB of LDS of a CU the variant falls back to the double-buffered tile.
-template <typename T, int KID, int BX, int BY, int RY>
-constexpr bool ring_fits() {
-  constexpr int R = kradius(KID);
-  return (size_t)(R + 2) * (BY * RY + 2 * R) * (64 * BX + 2 * R) * sizeof(T) <= 160 * 1024;
-}
-
-template <typename T, typename C, int KID, bool FAST, bool ROCS>
-static hipError_t launch_sweep_t(const SweepArgs<T, C> &a, const LaunchCfg &lc, hipStream_t st) {
-  constexpr int BY = Tile<T, KID>::BY, RY = Tile<T, KID>::RY;
-  constexpr int W = std::is_same<T, double>::value ? 3 : 4;
-  if constexpr (KID == 0 || KID == 1 || KID == 5) {
-    switch (lc.variant) {
-      case 1: return launch_cfg<T, C, KID, FAST, ROCS, 8, 4, 0>(a, lc, st);
-      case 2: return launch_cfg<T, C, KID, FAST, ROCS, 8, 2, 0>(a, lc, st);
-      case 3: return launch_cfg<T, C, KID, FAST, ROCS, 8, 4, W>(a, lc, st);
-      case 4: return launch_cfg<T, C, KID, FAST, ROCS, 4, 4, W>(a, lc, st);
-      case 5: return launch_cfg<T, C, KID, FAST, ROCS, 8, 4, W, true>(a, lc, st);
-      case 6: return launch_cfg<T, C, KID, FAST, ROCS, 8, 2, 0, true>(a, lc, st);
-      case 7: return launch_cfg<T, C, KID, FAST, ROCS, 4, 4, W, true>(a, lc, st);
-      case 8: return launch_cfg<T, C, KID, FAST, ROCS, 4, 4, 0, false, 2>(a, lc, st);
-      case 9: return launch_cfg<T, C, KID, FAST, ROCS, 8, 4, 0, false, 2>(a, lc, st);
-      case 10: return launch_cfg<T, C, KID, FAST, ROCS, 4, 4, 0, true, 2>(a, lc, st);
-      case 11: return launch_cfg<T, C, KID, FAST, ROCS, 4, 4, W, false, 2>(a, lc, st);
-      case 12: return launch_cfg<T, C, KID, FAST, ROCS, 2, 4, 0, true, 2>(a, lc, st);
-      case 13: return launch_cfg<T, C, KID, FAST, ROCS, 8, 2, 0, true, 2>(a, lc, st);
-      case 14: return launch_cfg<T, C, KID, FAST, ROCS, 4, 4, W, true, 2>(a, lc, st);
-      case 15: return launch_cfg<T, C, KID, FAST, ROCS, 2, 4, 0, ring_fits<T, KID, 4, 2, 4>(), 4>(a, lc, st);
-      default: break;
-    }
-  }
-  return launch_cfg<T, C, KID, FAST, ROCS, BY, RY, 0>(a, lc, st);
-}
-
-template <typename T, typename C>
-static hipError_t launch_sweep_k(int kid, bool fast, bool rocs, const SweepArgs<T, C> &a,
-                                 const LaunchCfg &lc, hipStream_t st) {
-  switch (kid) {
-    case 0:
-      if constexpr (std::is_same<T, float>::value) {
-        if (fast) {
-          if (rocs) return launch_sweep_t<T, C, 0, true, true>(a, lc, st);
-          return launch_sweep_t<T, C, 0, true, false>(a, lc, st);
-        }
-      }
-      if (rocs) return launch_sweep_t<T, C, 0, false, true>(a, lc, st);
-      return launch_sweep_t<T, C, 0, false, false>(a, lc, st);
-    case 1: return launch_sweep_t<T, C, 1, false, false>(a, lc, st);
-    case 2: return launch_sweep_t<T, C, 2, false, false>(a, lc, st);
-    case 3: return launch_sweep_t<T, C, 3, false, false>(a, lc, st);
-    case 4: return launch_sweep_t<T, C, 4, false, false>(a, lc, st);
-    case 5: return launch_sweep_t<T, C, 5, false, false>(a, lc, st);
-  }
-  return hipErrorInvalidValue;
 }
 
 // One sweep u <- f(v) with the handle's configuration.
@@ -188,7 +121,7 @@ hipError_t sweep_fields(const Problem &P, void *u, const void *v, const void *sr
     a.src = (const double *)src;
     a.src_k = src_idx < 0 ? -1 : (int)(src_idx / g.sz);
     a.src_off = src_idx < 0 ? 0 : (int)(src_idx % g.sz);
-    return launch_sweep_k<double, double>(P.kernel, false, rocs, a, lc, st);
+    return launch_sweep_f64(P.kernel, false, rocs, a, lc, st, streaming_sweep(P));
   }
   if (bf16) {
     SweepArgs<float, __hip_bfloat16> a{};
@@ -205,7 +138,7 @@ hipError_t sweep_fields(const Problem &P, void *u, const void *v, const void *sr
     a.src = (const float *)src;
     a.src_k = src_idx < 0 ? -1 : (int)(src_idx / g.sz);
     a.src_off = src_idx < 0 ? 0 : (int)(src_idx % g.sz);
-    return launch_sweep_k<float, __hip_bfloat16>(P.kernel, fast, rocs, a, lc, st);
+    return launch_sweep_bf16(P.kernel, fast, rocs, a, lc, st, streaming_sweep(P));
   }
   SweepArgs<float, float> a{};
   a.u = (float *)u;
@@ -221,7 +154,7 @@ hipError_t sweep_fields(const Problem &P, void *u, const void *v, const void *sr
   a.src = (const float *)src;
   a.src_k = src_idx < 0 ? -1 : (int)(src_idx / g.sz);
   a.src_off = src_idx < 0 ? 0 : (int)(src_idx % g.sz);
-  return launch_sweep_k<float, float>(P.kernel, fast, rocs, a, lc, st);
+  return launch_sweep_f32(P.kernel, fast, rocs, a, lc, st, streaming_sweep(P));
 }
 
 }  // namespace gr
@@ -907,7 +840,7 @@ hipError_t sweep_box(const Problem &P, const Geom &g, void *u, const void *v, co
     for (int r = 0; r < 5; ++r) a.c[r] = P.c[r];
     a.g = g;
     a.src_k = -1;
-    return launch_sweep_k<double, double>(P.kernel, false, false, a, lc, st);
+    return launch_sweep_f64(P.kernel, false, false, a, lc, st, false);
   }
   SweepArgs<float, float> a{};
   a.u = (float *)u;
@@ -918,7 +851,7 @@ hipError_t sweep_box(const Problem &P, const Geom &g, void *u, const void *v, co
   for (int r = 0; r < 5; ++r) a.c[r] = (float)P.c[r];
   a.g = g;
   a.src_k = -1;
-  return launch_sweep_k<float, float>(P.kernel, false, false, a, lc, st);
+  return launch_sweep_f32(P.kernel, false, false, a, lc, st, false);
 }
 }  // namespace gr
 

@@ -31,8 +31,14 @@
 #include <hip/hip_bf16.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #ifndef GR_FAST_ND
 #define GR_FAST_ND 2
+#endif
+
+#ifndef GR_STREAMING_HINTS
+#define GR_STREAMING_HINTS 1
 #endif
 
 namespace gr {
@@ -78,14 +84,22 @@ struct SweepArgs {
   int src_off;
 };
 
-template <typename T, typename C> __device__ __forceinline__ T ld_coef(const C *p) {
-  return static_cast<T>(*p);
+// Coefficient and roc2 fields are read once per sweep: with NT, loads carry the
+// non-temporal hint (see ldb_stream).  bf16 values widen exactly (the bits move to
+// the top half).
+template <bool NT, typename X>
+__device__ __forceinline__ X ld_once(const X *p) {
+  if constexpr (NT && GR_STREAMING_HINTS) return __builtin_nontemporal_load(p);
+  return *p;
 }
-template <> __device__ __forceinline__ float ld_coef<float, __hip_bfloat16>(const __hip_bfloat16 *p) {
-  return __bfloat162float(*p);
-}
-template <> __device__ __forceinline__ double ld_coef<double, __hip_bfloat16>(const __hip_bfloat16 *p) {
-  return static_cast<double>(__bfloat162float(*p));
+template <typename T, typename C, bool NT>
+__device__ __forceinline__ T ld_coef(const C *p) {
+  if constexpr (std::is_same<C, __hip_bfloat16>::value) {
+    const unsigned short b = ld_once<NT>(reinterpret_cast<const unsigned short *>(p));
+    return static_cast<T>(__uint_as_float((unsigned)b << 16));
+  } else {
+    return static_cast<T>(ld_once<NT>(p));
+  }
 }
 
 // Loads/stores through a uniform base and a 32-bit byte offset: the compiler then
@@ -98,6 +112,26 @@ __device__ __forceinline__ X ldb(const void *base, unsigned boff) {
 template <typename X>
 __device__ __forceinline__ void stb(void *base, unsigned boff, X v) {
   *reinterpret_cast<X *>(reinterpret_cast<char *>(base) + boff) = v;
+}
+// Streaming variants for data touched once per sweep (the output plane and the
+// u^(n-1) operand).  When a sweep's working set is far larger than the caches
+// (NT), non-temporal hints keep these from displacing the halo lines that
+// neighbouring tiles still read from L2; a small grid that stays in the cache
+// between steps loads and stores normally.  The choice is a template parameter: a runtime
+// choice between the two forms of one access lets the compiler merge them and drop
+// the hint.
+template <typename X, bool NT>
+__device__ __forceinline__ X ldb_stream(const void *base, unsigned boff) {
+  if constexpr (NT && GR_STREAMING_HINTS)
+    return __builtin_nontemporal_load(reinterpret_cast<const X *>(reinterpret_cast<const char *>(base) + boff));
+  return ldb<X>(base, boff);
+}
+template <typename X, bool NT>
+__device__ __forceinline__ void stb_stream(void *base, unsigned boff, X v) {
+  if constexpr (NT && GR_STREAMING_HINTS)
+    __builtin_nontemporal_store(v, reinterpret_cast<X *>(reinterpret_cast<char *>(base) + boff));
+  else
+    stb<X>(base, boff, v);
 }
 
 // Map a linear workgroup id to a logical tile so that each of the 8 XCDs (which
@@ -197,7 +231,8 @@ struct Tile {
   static constexpr int RY = (KID == 4) ? 2 : 4;
 };
 
-template <typename T, typename C, int KID, int BY, int RY, bool FAST, bool ROCS, bool RING, int ND, int BX = 1>
+template <typename T, typename C, int KID, int BY, int RY, bool FAST, bool ROCS, bool RING, int ND, int BX = 1,
+          bool STRM = false>
 struct SweepTile;
 
 // MINW > 0 asks the compiler for at least MINW waves per SIMD (caps VGPRs at 512/MINW).
@@ -206,10 +241,10 @@ struct SweepTile;
 // the neighbouring tiles fetch it too and it is still in L2; otherwise the LDS is
 // double buffered and the halo is fetched one plane ahead.
 template <typename T, typename C, int KID, int BY, int RY, bool FAST, bool ROCS, int MINW, bool RING, int ND = kFastND,
-          int BX = 1>
+          int BX = 1, bool STRM = false>
 __global__ __launch_bounds__(64 * BX * BY) __attribute__((amdgpu_waves_per_eu(MINW > 0 ? MINW : 1)))
 void sweep_kernel(const SweepArgs<T, C> a) {
-  using ST = SweepTile<T, C, KID, BY, RY, FAST, ROCS, RING, ND, BX>;
+  using ST = SweepTile<T, C, KID, BY, RY, FAST, ROCS, RING, ND, BX, STRM>;
   __shared__ typename ST::Lds lds;
   const int nb = a.tiles_x * a.tiles_y * a.nzc;
   const int lb = a.xcd_order ? xcd_remap(blockIdx.x, nb) : (int)blockIdx.x;
@@ -221,7 +256,7 @@ void sweep_kernel(const SweepArgs<T, C> a) {
 
 // One tile of one sweep: the z-streaming body shared by the single-grid kernel
 // and the batched persistent kernel.
-template <typename T, typename C, int KID, int BY, int RY, bool FAST, bool ROCS, bool RING, int ND, int BX>
+template <typename T, typename C, int KID, int BY, int RY, bool FAST, bool ROCS, bool RING, int ND, int BX, bool STRM>
 struct SweepTile {
   static constexpr int R = kradius(KID);
   static constexpr int TX = 64 * BX, TY = BY * RY;
@@ -231,8 +266,8 @@ struct SweepTile {
   __device__ static __forceinline__ void run(const SweepArgs<T, C> &a, int tile_x, int tile_y, int zc, Lds &lds);
 };
 
-template <typename T, typename C, int KID, int BY, int RY, bool FAST, bool ROCS, bool RING, int ND, int BX>
-__device__ __forceinline__ void SweepTile<T, C, KID, BY, RY, FAST, ROCS, RING, ND, BX>::run(const SweepArgs<T, C> &a,
+template <typename T, typename C, int KID, int BY, int RY, bool FAST, bool ROCS, bool RING, int ND, int BX, bool STRM>
+__device__ __forceinline__ void SweepTile<T, C, KID, BY, RY, FAST, ROCS, RING, ND, BX, STRM>::run(const SweepArgs<T, C> &a,
                                                                                    int tile_x, int tile_y, int zc,
                                                                                    Lds &lds) {
   constexpr int Q = 2 * R + 1;
@@ -333,12 +368,12 @@ __device__ __forceinline__ void SweepTile<T, C, KID, BY, RY, FAST, ROCS, RING, N
 #pragma unroll
         for (int m = 0; m < RY; ++m) {
           if constexpr (KID == 0) {
-            uo[m] = ldb<T>(up, off[m]);
-            if constexpr (!ROCS) r2[m] = ld_coef<T, C>(a.roc2 + pk + off[m] / (unsigned)sizeof(T));
+            uo[m] = ldb_stream<T, STRM>(up, off[m]);
+            if constexpr (!ROCS) r2[m] = ld_coef<T, C, STRM>(a.roc2 + pk + off[m] / (unsigned)sizeof(T));
           }
 #pragma unroll
           for (int n = 0; n < NC; ++n)
-            cf[m][n] = ld_coef<T, C>(a.coef + n * a.cstride + pk + off[m] / (unsigned)sizeof(T));
+            cf[m][n] = ld_coef<T, C, STRM>(a.coef + n * a.cstride + pk + off[m] / (unsigned)sizeof(T));
         }
       }
       // plane k+R+1 into the slot freed by plane k-R, halo ring of plane k+1
@@ -392,7 +427,7 @@ __device__ __forceinline__ void SweepTile<T, C, KID, BY, RY, FAST, ROCS, RING, N
                 cf[m][4] * yn(1) + cf[m][5] * zq[0] + cf[m][6] * zq[2];
         }
         if (src_plane && off[m] == (unsigned)a.src_off * (unsigned)sizeof(T)) out = out + *a.src;
-        if (yout[m]) stb<T>(uk, off[m], out);
+        if (yout[m]) stb_stream<T, STRM>(uk, off[m], out);
       }
 #pragma unroll
       for (int m = 0; m < RY; ++m) q[m][p] = nq[m];

@@ -225,6 +225,7 @@ __global__ __launch_bounds__(NT) void tb_kernel(const TbArgs<T> a) {
               for (int d = -R; d <= R; ++d) zq[R + d] = zn(d);
               const T v0 = zq[R];
               if constexpr (KID == 0) {
+                // roc2 and coefficients are read again by every level: keep them cached
                 const T roc = ROCS ? a.roc2s : a.roc2[(long long)p * a.sz + off[j] / sizeof(T)];
                 out = Op<T, 0, FAST>::update(v0, uo[j], roc,
                                              Op<T, 0, FAST>::template lap<kFastND>(c, a.csum, v0, zq, xn, yn));
