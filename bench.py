@@ -127,17 +127,19 @@ def _timed_steps(st, napply):
     return time.perf_counter() - t0
 
 
-def extra_first_order(kernel, n, nt, coef=None, fp32=True, graph=False):
+def extra_first_order(kernel, n, nt, coef=None, fp32=True, graph=False, bf16=False):
     """girih kernels 1 / 5: FP64 (bit-exact path) and optionally FP32 on the same
     seeded input, autotuned; FP32 error vs FP64 reduced on the device.  graph=True
     replays the step loop from a HIP graph (captured by an untimed first run)."""
     from girih_amd import gpu
     napply = 2 * ((nt + 1) // 2)  # girih's SB/MWD application count for even nt
     out = {}
-    runs = [("f64", gpu.GR_F64)] + ([("f32", gpu.GR_F32)] if fp32 else [])
+    runs = [("f64", gpu.GR_F64, 0)] + ([("f32", gpu.GR_F32, 0)] if fp32 else [])
+    if bf16:  # FP32 fields, coefficient arrays stored as bf16 (same seed, rounded to nearest even)
+        runs.append(("f32_bf16coef", gpu.GR_F32, gpu.GR_COEF_BF16))
     keep = {}
-    for name, dtype in runs:
-        st = gpu.Stepper(kernel, n, n, n, dtype=dtype, coef=coef)
+    for name, dtype, flags in runs:
+        st = gpu.Stepper(kernel, n, n, n, dtype=dtype, flags=flags, coef=coef)
         if gpu.NUM_COEF_ARRAYS[kernel]:
             st.init_coef_uniform(5, 0.3, 0.5, 0.0, 0.1)
         st.init_fields(1, 12345)
@@ -149,7 +151,8 @@ def extra_first_order(kernel, n, nt, coef=None, fp32=True, graph=False):
         st.init_fields(1, 12345)
         t = _timed_steps(st, napply)
         s = st.stats()
-        b_sweep = (2 + gpu.NUM_COEF_ARRAYS[kernel]) * (8 if dtype == gpu.GR_F64 else 4)
+        b_sweep = 2 * (8 if dtype == gpu.GR_F64 else 4) + gpu.NUM_COEF_ARRAYS[kernel] * (
+            2 if flags & gpu.GR_COEF_BF16 else (8 if dtype == gpu.GR_F64 else 4))
         gl = n ** 3 * napply / t / 1e9
         out[name] = {"glups": round(gl, 2), "ms_per_step": round(t * 1e3 / napply, 4),
                      "sweep_roofline_frac": round(gl * b_sweep / HBM_PEAK_GBS, 4), "bytes_per_lup_sweep": b_sweep,
@@ -157,15 +160,19 @@ def extra_first_order(kernel, n, nt, coef=None, fp32=True, graph=False):
                                                                                      tuned["tile_y"]],
                      "launches": s["last_launches"], "graph": graph}
         keep[name] = st
-    if "f32" in keep:
-        c = keep["f32"].compare(1, keep["f64"], 1)
-        out["f32"]["rel_l2_vs_f64"] = c["rel_l2"]
-        out["f32"]["f64_max_abs_after_run"] = c["max_ref"]  # below FP32's range when the stencil contracts
+    lowp = [k for k in keep if k != "f64"]
+    for k in lowp:
+        c = keep[k].compare(1, keep["f64"], 1)
+        out[k]["rel_l2_vs_f64"] = c["rel_l2"]
+        out[k]["f64_max_abs_after_run"] = c["max_ref"]  # below FP32's range when the stencil contracts
+    if lowp:
         for st in keep.values():  # the error while the field is still representable in FP32
             st.init_fields(1, 12345)
             st.step(100)
-        c = keep["f32"].compare(1, keep["f64"], 1)
-        out["f32"]["rel_l2_vs_f64_after_100"] = c["rel_l2"]
+        for k in lowp:
+            out[k]["rel_l2_vs_f64_after_100"] = keep[k].compare(1, keep["f64"], 1)["rel_l2"]
+        if "f32_bf16coef" in out:
+            out["f32_bf16coef"]["note"] = "error includes the bf16 rounding of the coefficients"
     for st in keep.values():
         st.close()
     return out
@@ -393,7 +400,7 @@ def main():
         print("bench: extras", file=sys.stderr, flush=True)
         out["extras"] = {
             "jacobi7_f64_128": extra_first_order(1, 128, 100, coef=[0.4, 0.1], fp32=False, graph=True),
-            "var7_nosym_512": extra_first_order(5, 512, 500),
+            "var7_nosym_512": extra_first_order(5, 512, 500, bf16=True),
             "batch16_wave25_bf16": extra_batch(500),
             "adjoint_wave25_400": extra_adjoint(),
         }

@@ -320,6 +320,11 @@ extern "C" int gr_destroy(gr_handle *h) {
   return GR_OK;
 }
 
+__global__ void to_bf16_kernel(const float *src, __hip_bfloat16 *dst, long long n) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) dst[t] = __float2bfloat16(src[t]);  // round to nearest even
+}
+
 // host girih layout <-> device layout: both store rows (k*nny + j) at a constant pitch
 static int copy_field(gr_handle *h, const int ld[3], void *dev_base, size_t dev_es, const void *host,
                       size_t host_es, bool to_dev) {
@@ -351,18 +356,42 @@ extern "C" int gr_upload(gr_handle *h, const int ld[3], const void *U1, const vo
   if (rc) return rc;
   const Problem &P = h->P;
   const size_t es = elem_size(P.dtype);
-  if (P.flags & GR_COEF_BF16)
-    if ((P.kernel == 0 && U3 && !(P.flags & GR_ROC2_SCALAR)) || (P.ncoef > 0 && coef))
-      return fail(GR_EUNSUPPORTED, "gr_upload: bf16 coefficient fields are filled on the device (gr_init_*)");
   if (U1 && (rc = copy_field(h, ld, ubase(h, 0), es, U1, es, true))) return rc;
   if (U2 && (rc = copy_field(h, ld, ubase(h, 1), es, U2, es, true))) return rc;
-  if (U3 && P.roc2 && (rc = copy_field(h, ld, P.roc2, es, U3, es, true))) return rc;
-  if (coef && P.coef) {
-    const size_t hstride = (size_t)ld[0] * ld[1] * ld[2] * es;
-    const size_t dstride = (size_t)P.field_elems * es;
-    for (int m = 0; m < P.ncoef; ++m)
-      if ((rc = copy_field(h, ld, (char *)P.coef + m * dstride, es, (const char *)coef + m * hstride, es, true)))
-        return rc;
+  const size_t hstride = (size_t)ld[0] * ld[1] * ld[2] * es;
+  if (P.flags & GR_COEF_BF16) {
+    // bf16 roc2 / coefficient fields: FP32 host arrays staged through one FP32
+    // field on the device and rounded to nearest even
+    const bool want_r = U3 && P.roc2, want_c = coef && P.coef;
+    if (want_r || want_c) {
+      void *tmp = nullptr;
+      HIP_TRY(hipMalloc(&tmp, (size_t)P.field_elems * sizeof(float)));
+      float *tbase = (float *)tmp + P.xoff;
+      auto one = [&](const void *host, __hip_bfloat16 *dst) -> int {
+        int r = copy_field(h, ld, tbase, sizeof(float), host, es, true);
+        if (r) return r;
+        const unsigned nb = (unsigned)((P.field_elems + 255) / 256);
+        hipLaunchKernelGGL(to_bf16_kernel, dim3(nb), dim3(256), 0, h->stream, (const float *)tmp,
+                           dst - P.xoff, P.field_elems);
+        const hipError_t e = hipGetLastError();
+        return e == hipSuccess ? GR_OK : fail(GR_EHIP, "bf16 conversion: %s", hipGetErrorString(e));
+      };
+      if (want_r) rc = one(U3, (__hip_bfloat16 *)P.roc2);
+      for (int m = 0; rc == GR_OK && want_c && m < P.ncoef; ++m)
+        rc = one((const char *)coef + m * hstride, (__hip_bfloat16 *)P.coef + (size_t)m * P.field_elems);
+      const hipError_t e = hipStreamSynchronize(h->stream);
+      (void)hipFree(tmp);
+      if (rc) return rc;
+      if (e != hipSuccess) return fail(GR_EHIP, "gr_upload: %s", hipGetErrorString(e));
+    }
+  } else {
+    if (U3 && P.roc2 && (rc = copy_field(h, ld, P.roc2, es, U3, es, true))) return rc;
+    if (coef && P.coef) {
+      const size_t dstride = (size_t)P.field_elems * es;
+      for (int m = 0; m < P.ncoef; ++m)
+        if ((rc = copy_field(h, ld, (char *)P.coef + m * dstride, es, (const char *)coef + m * hstride, es, true)))
+          return rc;
+    }
   }
   HIP_TRY(hipStreamSynchronize(h->stream));
   state_reset(h);

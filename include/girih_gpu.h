@@ -94,8 +94,9 @@ const char *gr_last_error(void);
  * (girih p->ldomain_shape, utils.c:372-374; nnx may include padding); x is fastest.
  * U3 is roc2 (kernel 0 without GR_ROC2_SCALAR), coef holds girih's SoA coefficient
  * arrays (coef[m*nnx*nny*nnz + idx], stencils.h:32) for kernels 2-5.  Any pointer
- * the kernel does not use may be NULL.  Element type follows desc.dtype.
- * Resets the step parity (the next application writes U1). */
+ * the kernel does not use may be NULL.  Element type follows desc.dtype; with
+ * GR_COEF_BF16, U3 and coef are FP32 host arrays rounded to bf16 (nearest even) on
+ * the device.  Resets the step parity (the next application writes U1). */
 int gr_upload(gr_handle *h, const int ldomain_shape[3], const void *U1, const void *U2,
               const void *U3, const void *coef);
 /* Copy U1 and U2 back into girih-layout host arrays (either may be NULL). */

@@ -155,6 +155,35 @@ def test_every_tile_variant_fast_identical():
             np.testing.assert_array_equal(out[1], ref[1], err_msg="variant %d" % var)
 
 
+def _bf16_round(a):
+    """float32 array rounded to bfloat16 values (nearest even), kept as float32."""
+    b = np.ascontiguousarray(a, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    b = ((b + 0x7FFF + ((b >> 16) & 1)) >> 16) << 16
+    return b.astype(np.uint32).view(np.float32)
+
+
+@pytest.mark.parametrize("kernel", [0, 2, 3, 4, 5])
+def test_bf16_coefficient_fields_bitexact(kernel):
+    """GR_COEF_BF16 stores roc2 / coefficient arrays as bfloat16 and computes in
+    FP32 in girih's operand order: bit-identical to the FP32 oracle run on the same
+    bf16-rounded coefficients."""
+    nx, ny, nz = (70, 21, 13) if orc.RADIUS[kernel] == 4 else (77, 35, 12)
+    u, v, roc2, coef = _random_problem(kernel, nx, ny, nz, np.float32, 300 + kernel)
+    if roc2 is not None:
+        roc2 = _bf16_round(roc2)
+    if orc.NUM_COEF_ARRAYS[kernel]:
+        coef = _bf16_round(coef)
+    with gpu.Stepper(kernel, nx, ny, nz, dtype=gpu.GR_F32, flags=gpu.GR_COEF_BF16, coef=_consts(coef, kernel)) as st:
+        st.upload(_pad(u), _pad(v), _pad(roc2) if roc2 is not None else None, _pad_coef(coef, kernel))
+        st.step(5)
+        U1, U2 = st.download(_pad(u).shape)
+    orc.run(kernel, coef, u, v, roc2, nt=4)
+    orc.sweep(kernel, coef, u, v, roc2)
+    nnx = u.shape[2]
+    np.testing.assert_array_equal(U1[:, :, :nnx], u)
+    np.testing.assert_array_equal(U2[:, :, :nnx], v)
+
+
 def test_device_random_coefficients_do_not_depend_on_precision():
     """gr_init_coef_uniform and gr_init_fields(mode 1) draw from logical cell indices:
     an FP32 and an FP64 problem with the same seeds hold the same values up to
