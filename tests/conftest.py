@@ -10,7 +10,7 @@ for p in (REPO, os.path.join(REPO, "oracle")):
 
 
 def pytest_configure(config):
-    config.addinivalue_line("markers", "gpu: needs an MI355X (run with -m gpu on the GPU box)")
+    config.addinivalue_line("markers", "gpu: needs an MI355X GPU (select with -m gpu)")
     config.addinivalue_line("markers", "slow: long CPU test")
 
 
