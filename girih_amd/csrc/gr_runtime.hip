@@ -767,10 +767,12 @@ extern "C" int gr_autotune(gr_handle *h, int steps, gr_stats *best) {
       if (!ok) continue;
       const int tl = depths[i];
       const int npass = (steps + tl - 1) / tl;
-      for (int zc : zchunks) {
+      for (int zc : zchunks)
+      for (int order = 0; order < 2; ++order) {  // XCD-contiguous order keeps the per-level re-reads in one L2
         LaunchCfg lc = best_lc;
         lc.t_block = tl;
         lc.zchunk = zc;
+        lc.xcd_order = order;
         auto pass = [&]() {
           if (P.kernel == 0) return tblock_pass(P, tl, ubase(h, 0), ubase(h, 1), a, b, nullptr, -1, lc, h->stream);
           return tblock_pass(P, tl, nullptr, b, nullptr, a, nullptr, -1, lc, h->stream);
