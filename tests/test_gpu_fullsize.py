@@ -1,0 +1,95 @@
+"""Parity at the benchmark sizes, and at the smallest grids.
+
+* 7-point Jacobi, 128^3, FP64, 100 steps: bit-identical to the CPU oracle;
+* 7-point var-coef, 512^3, FP64, 2 steps: bit-identical to the CPU oracle (the
+  fields and coefficient arrays span several GB: 64-bit plane offsets);
+* 25-point wave, 1024^3, FP64, through the Level-2 entry point on torch memory:
+  a field symmetric under every axis reflection stays symmetric bit for bit
+  (floating-point addition is commutative, so mirrored cells evaluate the same
+  sums), which the oracle cannot check at this size in reasonable time;
+* 1^3 .. 3^3 grids for every kernel: bit-identical to the oracle.
+"""
+import numpy as np
+import pytest
+
+import oracle as orc
+from girih_amd import gpu, physics
+
+from test_gpu_parity import _consts, _pad, _pad_coef, _random_problem
+
+pytestmark = pytest.mark.gpu
+
+
+def test_jacobi_128_100_steps_bitexact():
+    kernel, n = 1, 128
+    u, v, _, coef = _random_problem(kernel, n, n, n, np.float64, 128)
+    with gpu.Stepper(kernel, n, n, n, dtype=gpu.GR_F64, coef=_consts(coef, kernel)) as st:
+        st.upload(_pad(u), _pad(v))
+        st.autotune(4)
+        st.set_graph(True)
+        assert st.run_ts(0, 100) == 100
+        U1, U2 = st.download(_pad(u).shape)
+    orc.run(kernel, coef, u, v, None, nt=100)
+    np.testing.assert_array_equal(U1[:, :, :u.shape[2]], u)
+    np.testing.assert_array_equal(U2[:, :, :u.shape[2]], v)
+
+
+def test_var7_512_bitexact():
+    kernel, n = 5, 512
+    rng = np.random.default_rng(512)
+    shp = orc.shape_of(kernel, n, n, n)
+    u = np.zeros(shp)
+    v = np.zeros(shp)
+    v[1:-1, 1:-1, 1:-1] = rng.uniform(0, 1, (n, n, n))
+    coef = rng.uniform(0, 0.1, (7,) + shp)
+    coef[0] += 0.3
+    with gpu.Stepper(kernel, n, n, n, dtype=gpu.GR_F64) as st:
+        st.upload(_pad(u), _pad(v), None, _pad_coef(coef, kernel))
+        st.set_config(zchunk=128, t_block=2)  # one fused pass and one sweep
+        st.step(3)
+        U1, U2 = st.download(_pad(u).shape)
+    orc.run(kernel, coef, u, v, None, nt=2)
+    orc.sweep(kernel, coef, u, v)
+    np.testing.assert_array_equal(U1[:, :, :shp[2]], u)
+    np.testing.assert_array_equal(U2[:, :, :shp[2]], v)
+
+
+def test_wave_1024_mirror_symmetry_level2():
+    torch = pytest.importorskip("torch")
+    n, R = 1024, 4
+    m = n + 2 * R  # 1032, already a multiple of 8: no padding
+    c = (m - 1) / 2.0
+    ax = torch.arange(m, dtype=torch.float64, device="cuda") - c
+    g = ax * ax
+    u = torch.exp(-(g[:, None, None] + g[None, :, None] + g[None, None, :]) / (2 * 90.0 ** 2))
+    u[:R], u[-R:], u[:, :R], u[:, -R:], u[:, :, :R], u[:, :, -R:] = 0, 0, 0, 0, 0, 0  # girih's zero halo
+    v = u.clone()
+    roc2 = torch.full_like(u, 0.16)
+    coef = np.array(physics.fd8_coef())
+    shape = (m, m, m)
+    box = (R, R, R, n + R, n + R, n + R)
+    for s in range(6):
+        a, b = (u, v) if s % 2 == 0 else (v, u)
+        gpu.sweep_dev(0, gpu.GR_F64, shape, box, coef, a.data_ptr(), b.data_ptr(), roc2.data_ptr())
+    torch.cuda.synchronize()
+    for f in (u, v):
+        for d in range(3):
+            assert torch.equal(f, f.flip(d)), "axis %d" % d
+    assert float(u.abs().max()) > 0.1
+    del u, v, roc2
+    torch.cuda.empty_cache()
+
+
+@pytest.mark.parametrize("kernel", [0, 1, 2, 3, 4, 5])
+def test_smallest_grids_bitexact(kernel):
+    for n in (1, 2, 3):
+        for prec, dt, dtype in (("dp", np.float64, gpu.GR_F64), ("sp", np.float32, gpu.GR_F32)):
+            u, v, roc2, coef = _random_problem(kernel, n, n + 1, n, dt, 10 * n + kernel)
+            with gpu.Stepper(kernel, n, n + 1, n, dtype=dtype, coef=_consts(coef, kernel)) as st:
+                st.upload(_pad(u), _pad(v), _pad(roc2) if roc2 is not None else None, _pad_coef(coef, kernel))
+                st.step(3)
+                U1, U2 = st.download(_pad(u).shape)
+            orc.run(kernel, coef, u, v, roc2, nt=2)
+            orc.sweep(kernel, coef, u, v, roc2)
+            np.testing.assert_array_equal(U1[:, :, :u.shape[2]], u, err_msg="%s %d" % (prec, n))
+            np.testing.assert_array_equal(U2[:, :, :u.shape[2]], v, err_msg="%s %d" % (prec, n))
