@@ -13,18 +13,18 @@
 // Level t is computed over the tile plus an xy margin of R*(TL - t) cells (the
 // overlapped ghost zone that makes the tile independent of its neighbours).  A
 // lane owns columns of the level-1 region; for each level it keeps a register
-// queue of the 2R+1 most recent planes (the z neighbours of the next level and,
-// for the second-order wave, the u^(n-1) term two levels up), and the current
-// plane of each level is staged in LDS for the xy neighbours of the next level.
+// queue of the 2R+1 most recent planes (the z neighbours of the next level), and
+// the current plane of each level is staged in LDS for the xy neighbours of the
+// next level.
 // Only the input field(s) are read from HBM and only the tile's output planes are
 // written, so a pass moves (in + out) words per cell for TL updates.
 //
 // Boundary cells (girih's halo, never updated) keep their input value at every
 // level; the host enables this path only when U1 and U2 hold the same halo.
 //
-// First-order kernels (1, 5): level t = f(level t-1); input Y, output Z.
-// Kernel 0 (u^(n+1) = 2u^n - u^(n-1) + roc2 Lap u^n): inputs X = u^(n-1),
-// Y = u^n; outputs ZO = level TL-1, ZN = level TL.
+// This kernel serves the first-order kernels 1 and 5: level t = f(level t-1),
+// input Y, output ZN.  Kernel 0 (second order in time) has its own two-step pass
+// in gr_wave2.hip.
 #include <hip/hip_runtime.h>
 
 #include "../../include/girih_gpu.h"
@@ -35,15 +35,11 @@ namespace gr {
 
 template <typename T>
 struct TbArgs {
-  const T *X;   // kernel 0: u^(n-1)
   const T *Y;   // level 0
-  T *ZO;        // kernel 0: level TL-1
   T *ZN;        // level TL
-  const T *roc2;
   const T *coef;
   long long cstride;
   T c[5];
-  T csum, roc2s;
   long long sy, sz;
   int nx, ny, nz;  // interior; boxes are [R, n+R)
   int tiles_x, tiles_y, xcd_order;
@@ -52,8 +48,9 @@ struct TbArgs {
   int src_k, src_off;
 };
 
-template <typename T, int KID, int NT, int TXO, int TYO, int TL, bool FAST, bool ROCS>
+template <typename T, int KID, int NT, int TXO, int TYO, int TL>
 __global__ __launch_bounds__(NT) void tb_kernel(const TbArgs<T> a) {
+  static_assert(KID == 1 || KID == 5, "first-order kernels only");
   constexpr int R = kradius(KID);
   constexpr int Q = 2 * R + 1;
   constexpr int NC = kncoef(KID);
@@ -191,19 +188,6 @@ __global__ __launch_bounds__(NT) void tb_kernel(const TbArgs<T> a) {
         T val[NCOL];
         if (live) {
           const T *Lp = lds[t - 1];
-          // kernel 0: u^(n+t-2) at plane p -- level t-2's queue, or X for t = 1
-          T uo[NCOL];
-          if constexpr (KID == 0) {
-            if (t == 1) {
-              const T *xp = a.X + (long long)p * a.sz;
-#pragma unroll
-              for (int j = 0; j < NCOL; ++j) uo[j] = ldb<T>(xp, off[j]);
-            } else {
-              const int so = ((ph - R * t) % Q + Q) % Q;  // plane p in level t-2's queue
-#pragma unroll
-              for (int j = 0; j < NCOL; ++j) uo[j] = qv[t - 2 >= 0 ? t - 2 : 0][j][so];
-            }
-          }
           const T *yp = planeY(p);
 #pragma unroll
           for (int j = 0; j < NCOL; ++j) {
@@ -224,14 +208,9 @@ __global__ __launch_bounds__(NT) void tb_kernel(const TbArgs<T> a) {
 #pragma unroll
               for (int d = -R; d <= R; ++d) zq[R + d] = zn(d);
               const T v0 = zq[R];
-              if constexpr (KID == 0) {
-                // roc2 and coefficients are read again by every level: keep them cached
-                const T roc = ROCS ? a.roc2s : a.roc2[(long long)p * a.sz + off[j] / sizeof(T)];
-                out = Op<T, 0, FAST>::update(v0, uo[j], roc,
-                                             Op<T, 0, FAST>::template lap<kFastND>(c, a.csum, v0, zq, xn, yn));
-              } else if constexpr (KID == 1) {  // stencils.c:73-78
+              if constexpr (KID == 1) {  // stencils.c:73-78
                 out = c[0] * v0 + c[1] * (xn(1) + xn(-1)) + c[1] * (yn(1) + yn(-1)) + c[1] * (zq[2] + zq[0]);
-              } else {  // KID == 5, stencils.c:193-201
+              } else {  // KID == 5, stencils.c:193-201; coefficients re-read by every level stay cached
                 const T *cp = a.coef + (long long)p * a.sz + off[j] / sizeof(T);
                 out = cp[0] * v0 + cp[a.cstride] * xn(-1) + cp[2 * a.cstride] * xn(1) +
                       cp[3 * a.cstride] * yn(-1) + cp[4 * a.cstride] * yn(1) + cp[5 * a.cstride] * zq[0] +
@@ -259,17 +238,11 @@ __global__ __launch_bounds__(NT) void tb_kernel(const TbArgs<T> a) {
           for (int j = 0; j < NCOL; ++j)
             if (colv[j]) Lt[lofs[j]] = qv[t < TL ? t : 0][j][sl];
         }
-        // outputs: level TL into ZN, level TL-1 into ZO (kernel 0) -- planes of the box
-        if (live && zin && p >= zs && p < ze) {
-          if (t == TL) {
+        // output: level TL into ZN -- planes of the box
+        if (t == TL && live && zin && p >= zs && p < ze) {
 #pragma unroll
-            for (int j = 0; j < NCOL; ++j)
-              if (outc[j]) stb<T>(a.ZN + (long long)p * a.sz, off[j], val[j]);
-          } else if (KID == 0 && t == TL - 1) {
-#pragma unroll
-            for (int j = 0; j < NCOL; ++j)
-              if (outc[j]) stb<T>(a.ZO + (long long)p * a.sz, off[j], val[j]);
-          }
+          for (int j = 0; j < NCOL; ++j)
+            if (outc[j]) stb<T>(a.ZN + (long long)p * a.sz, off[j], val[j]);
         }
         if (t < TL) __syncthreads();
       }
@@ -277,15 +250,14 @@ __global__ __launch_bounds__(NT) void tb_kernel(const TbArgs<T> a) {
   }
 }
 
-template <typename T, int KID, int NT, int TXO, int TYO, int TL, bool FAST, bool ROCS>
+template <typename T, int KID, int NT, int TXO, int TYO, int TL>
 static hipError_t launch_tb(const TbArgs<T> &a0, hipStream_t st) {
   TbArgs<T> a = a0;
   a.tiles_x = (a.nx + TXO - 1) / TXO;
   a.tiles_y = (a.ny + TYO - 1) / TYO;
   a.zchunk = (a.zchunk <= 0 || a.zchunk > a.nz) ? a.nz : a.zchunk;
   a.nzc = (a.nz + a.zchunk - 1) / a.zchunk;
-  hipLaunchKernelGGL((tb_kernel<T, KID, NT, TXO, TYO, TL, FAST, ROCS>), dim3(a.tiles_x * a.tiles_y * a.nzc), dim3(NT),
-                     0, st, a);
+  hipLaunchKernelGGL((tb_kernel<T, KID, NT, TXO, TYO, TL>), dim3(a.tiles_x * a.tiles_y * a.nzc), dim3(NT), 0, st, a);
   return hipGetLastError();
 }
 
@@ -301,7 +273,7 @@ int tb_depths(int kid, int dtype, int *out, int max) {
     add(4);
     add(6);
   } else if (kid == 0) {
-    add(2);
+    add(2);  // gr_wave2.hip
   }
   return n;
 }
@@ -309,69 +281,31 @@ int tb_depths(int kid, int dtype, int *out, int max) {
 // Tiles per kernel and precision, sized so the per-lane queues (levels x columns
 // x (2R+1) words) stay in registers without spilling.
 template <typename T>
-static hipError_t tb_dispatch(int kid, int tl, bool fast, bool rocs, const TbArgs<T> &a, hipStream_t st) {
+static hipError_t tb_dispatch(int kid, int tl, const TbArgs<T> &a, hipStream_t st) {
   constexpr bool F32 = std::is_same<T, float>::value;
-  constexpr int TYS = F32 ? 32 : 16;  // first-order kernels: 64 x TYS output tile, 512 lanes
+  constexpr int TYS = F32 ? 32 : 16;  // 64 x TYS output tile, 512 lanes
   if (kid == 1) {
-    if (tl == 2) return launch_tb<T, 1, 512, 64, TYS, 2, false, false>(a, st);
-    if (tl == 4) return launch_tb<T, 1, 512, 64, TYS, 4, false, false>(a, st);
-    if (tl == 6) return launch_tb<T, 1, 512, 64, 16, 6, false, false>(a, st);
+    if (tl == 2) return launch_tb<T, 1, 512, 64, TYS, 2>(a, st);
+    if (tl == 4) return launch_tb<T, 1, 512, 64, TYS, 4>(a, st);
+    if (tl == 6) return launch_tb<T, 1, 512, 64, 16, 6>(a, st);
   } else if (kid == 5) {
-    if (tl == 2) return launch_tb<T, 5, 512, 64, TYS, 2, false, false>(a, st);
-    if (tl == 4) return launch_tb<T, 5, 512, 64, TYS, 4, false, false>(a, st);
-    if (tl == 6) return launch_tb<T, 5, 512, 64, 16, 6, false, false>(a, st);
-  } else if (kid == 0 && tl == 2) {  // 64 x 16 output tile, 1024 lanes
-    if constexpr (F32) {
-      if (fast) {
-        if (rocs) return launch_tb<T, 0, 1024, 64, 16, 2, true, true>(a, st);
-        return launch_tb<T, 0, 1024, 64, 16, 2, true, false>(a, st);
-      }
-    }
-    if (rocs) return launch_tb<T, 0, 1024, 64, 16, 2, false, true>(a, st);
-    return launch_tb<T, 0, 1024, 64, 16, 2, false, false>(a, st);
+    if (tl == 2) return launch_tb<T, 5, 512, 64, TYS, 2>(a, st);
+    if (tl == 4) return launch_tb<T, 5, 512, 64, TYS, 4>(a, st);
+    if (tl == 6) return launch_tb<T, 5, 512, 64, 16, 6>(a, st);
   }
   return hipErrorInvalidValue;
 }
 
-// One fused pass of tl levels (see header comment for the field roles).
-hipError_t tblock_pass(const Problem &P, int tl, const void *X, const void *Y, void *ZO, void *ZN, const void *src,
-                       long long src_idx, const LaunchCfg &lc, hipStream_t st) {
-  const bool fast = (P.flags & GR_FAST) != 0, rocs = (P.flags & GR_ROC2_SCALAR) != 0;
+template <typename T>
+static hipError_t tb_pass_t(const Problem &P, int tl, const void *Y, void *ZN, const void *src, long long src_idx,
+                            const LaunchCfg &lc, hipStream_t st) {
   const long long sz = (long long)P.pitch * P.rows;
-  if (P.dtype == GR_F64) {
-    TbArgs<double> a{};
-    a.X = (const double *)X;
-    a.Y = (const double *)Y;
-    a.ZO = (double *)ZO;
-    a.ZN = (double *)ZN;
-    a.roc2 = (const double *)P.roc2;
-    a.coef = (const double *)P.coef;
-    a.cstride = P.field_elems;
-    for (int r = 0; r < 5; ++r) a.c[r] = P.c[r];
-    a.roc2s = P.roc2s;
-    a.sy = P.pitch;
-    a.sz = sz;
-    a.nx = P.nx;
-    a.ny = P.ny;
-    a.nz = P.nz;
-    a.xcd_order = lc.xcd_order;
-    a.zchunk = lc.zchunk;
-    a.src = (const double *)src;
-    a.src_k = src_idx < 0 ? -1 : (int)(src_idx / sz);
-    a.src_off = src_idx < 0 ? 0 : (int)(src_idx % sz);
-    return tb_dispatch<double>(P.kernel, tl, false, rocs, a, st);
-  }
-  TbArgs<float> a{};
-  a.X = (const float *)X;
-  a.Y = (const float *)Y;
-  a.ZO = (float *)ZO;
-  a.ZN = (float *)ZN;
-  a.roc2 = (const float *)P.roc2;
-  a.coef = (const float *)P.coef;
+  TbArgs<T> a{};
+  a.Y = (const T *)Y;
+  a.ZN = (T *)ZN;
+  a.coef = (const T *)P.coef;
   a.cstride = P.field_elems;
-  for (int r = 0; r < 5; ++r) a.c[r] = (float)P.c[r];
-  a.csum = fast_csum(P.c);
-  a.roc2s = (float)P.roc2s;
+  for (int r = 0; r < 5; ++r) a.c[r] = (T)P.c[r];
   a.sy = P.pitch;
   a.sz = sz;
   a.nx = P.nx;
@@ -379,10 +313,22 @@ hipError_t tblock_pass(const Problem &P, int tl, const void *X, const void *Y, v
   a.nz = P.nz;
   a.xcd_order = lc.xcd_order;
   a.zchunk = lc.zchunk;
-  a.src = (const float *)src;
+  a.src = (const T *)src;
   a.src_k = src_idx < 0 ? -1 : (int)(src_idx / sz);
   a.src_off = src_idx < 0 ? 0 : (int)(src_idx % sz);
-  return tb_dispatch<float>(P.kernel, tl, fast, rocs, a, st);
+  return tb_dispatch<T>(P.kernel, tl, a, st);
+}
+
+// One fused pass of tl levels.  Kernels 1 and 5: Y -> ZN (X, ZO unused).
+// Kernel 0 (tl = 2): (X, Y) = (u^(n-1), u^n) -> (ZO, ZN) = (u^(n+1), u^(n+2)).
+hipError_t tblock_pass(const Problem &P, int tl, const void *X, const void *Y, void *ZO, void *ZN, const void *src,
+                       long long src_idx, const LaunchCfg &lc, hipStream_t st) {
+  if (P.kernel == 0) {
+    if (tl != 2) return hipErrorInvalidValue;
+    return wave2_pass(P, X, Y, ZO, ZN, src, src_idx, lc, st);
+  }
+  if (P.dtype == GR_F64) return tb_pass_t<double>(P, tl, Y, ZN, src, src_idx, lc, st);
+  return tb_pass_t<float>(P, tl, Y, ZN, src, src_idx, lc, st);
 }
 
 }  // namespace gr

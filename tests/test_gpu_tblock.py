@@ -94,6 +94,55 @@ def test_fused_wave_with_source_fp64():
     np.testing.assert_array_equal(U2[:, :, :u.shape[2]], v)
 
 
+@pytest.mark.parametrize("prec", ["dp", "sp"])
+@pytest.mark.parametrize("roc2_kind", ["field", "scalar"])
+def test_fused_wave_tiles_and_source_on_tile_edges(prec, roc2_kind):
+    """Two-step kernel-0 passes over several xy tiles and z chunks, with the point
+    source on the first column and row of a tile (a margin cell of the neighbouring
+    tiles' first level): bit-identical to the oracle, with a roc2 field and with a
+    scalar roc2 (which the margin cells must use as well)."""
+    nx, ny, nz = 300, 50, 21
+    kernel = 0
+    dt, dtype = (np.float64, gpu.GR_F64) if prec == "dp" else (np.float32, gpu.GR_F32)
+    u, v, roc2, coef = _random_problem(kernel, nx, ny, nz, dt, 4242)
+    kw = {}
+    if roc2_kind == "scalar":
+        roc2 = np.full_like(roc2, 0.13)
+        kw = dict(flags=gpu.GR_ROC2_SCALAR, roc2=0.13)
+    pt = (4 + (64 if prec == "dp" else 128), 4 + 16, 4 + 10)  # output tiles are 64 x 16 (FP64), 128 x 16 (FP32)
+    src = np.cos(np.arange(10) * 0.7)
+    print("fused wave tiles", prec, roc2_kind, flush=True)
+    with gpu.Stepper(kernel, nx, ny, nz, dtype=dtype, coef=_consts(coef, kernel), **kw) as st:
+        st.upload(_pad(u), _pad(v), _pad(roc2) if roc2_kind == "field" else None)
+        st.set_source(*pt, src)
+        st.set_config(zchunk=5, t_block=2)
+        st.step(10)
+        assert st.stats()["last_launches"] == 5
+        U1, U2 = st.download(_pad(u).shape)
+    orc.run(kernel, coef, u, v, roc2, nt=10, src_idx=pt, src=src)
+    np.testing.assert_array_equal(U1[:, :, :u.shape[2]], u)
+    np.testing.assert_array_equal(U2[:, :, :u.shape[2]], v)
+
+
+def test_fused_fast_fp32_multi_tile_equals_single_sweeps():
+    """GR_FAST two-step passes with a scalar roc2 over several tiles, z chunks and a
+    source on a tile edge equal GR_FAST single sweeps bit for bit."""
+    nx, ny, nz = 260, 40, 30
+    kernel = 0
+    u, v, _, coef = _random_problem(kernel, nx, ny, nz, np.float32, 78)
+    outs = []
+    for depth in (1, 2):
+        with gpu.Stepper(kernel, nx, ny, nz, dtype=gpu.GR_F32, flags=gpu.GR_FAST | gpu.GR_ROC2_SCALAR, roc2=0.12,
+                         coef=list(coef[:5])) as st:
+            st.upload(_pad(u), _pad(v))
+            st.set_source(4 + 127, 4 + 15, 4 + 7, np.linspace(1.0, -1.0, 12))
+            st.set_config(zchunk=7, t_block=depth)
+            st.step(12)
+            outs.append(st.download(_pad(u).shape))
+    np.testing.assert_array_equal(outs[0][0], outs[1][0])
+    np.testing.assert_array_equal(outs[0][1], outs[1][1])
+
+
 def test_fused_fast_fp32_equals_single_sweep_fast():
     """GR_FAST fused passes use the same arithmetic as GR_FAST single sweeps."""
     nx, ny, nz = 80, 36, 24
