@@ -22,7 +22,7 @@ DRIVER_BINS = {"dp": os.path.join(DRIVER_DIR, "bin", "mwd_kernel_gpu_dp"),
                "sp": os.path.join(DRIVER_DIR, "bin", "mwd_kernel_gpu_sp")}
 
 HIP_SOURCES = ["gr_launch_f32.hip", "gr_launch_f64.hip", "gr_launch_bf16.hip", "gr_runtime.hip", "gr_tblock.hip",
-               "gr_wave2.hip", "gr_model.hip", "gr_batch.hip", "gr_adjoint.hip"]
+               "gr_wave2.hip", "gr_fused1.hip", "gr_model.hip", "gr_batch.hip", "gr_adjoint.hip"]
 ARCH = os.environ.get("GIRIH_OFFLOAD_ARCH", "gfx950")
 
 # -ffp-contract=off: the reference-order kernels must not fuse multiply-adds

@@ -41,6 +41,9 @@ hipError_t sweep_fields(const Problem &P, void *u, const void *v, const void *sr
 hipError_t tblock_pass(const Problem &P, int tl, const void *X, const void *Y, void *ZO, void *ZN, const void *src,
                        long long src_idx, const LaunchCfg &lc, hipStream_t st);
 int tb_depths(int kid, int dtype, int *out, int max);
+// tl kernel-1 steps per pass: Y -> Z.
+hipError_t jacobi_tb_pass(const Problem &P, int tl, const void *Y, void *Z, const void *src, long long src_idx,
+                          const LaunchCfg &lc, hipStream_t st);
 // Two kernel-0 steps per pass: (X, Y) = (u^(n-1), u^n) -> (ZO, ZN) = (u^(n+1), u^(n+2)).
 hipError_t wave2_pass(const Problem &P, const void *X, const void *Y, void *ZO, void *ZN, const void *src,
                       long long src_idx, const LaunchCfg &lc, hipStream_t st);

@@ -22,9 +22,9 @@
 // Boundary cells (girih's halo, never updated) keep their input value at every
 // level; the host enables this path only when U1 and U2 hold the same halo.
 //
-// This kernel serves the first-order kernels 1 and 5: level t = f(level t-1),
-// input Y, output ZN.  Kernel 0 (second order in time) has its own two-step pass
-// in gr_wave2.hip.
+// This kernel serves kernel 5 (seven coefficient arrays): level t = f(level t-1),
+// input Y, output ZN.  Kernel 1 has a leaner pass in gr_fused1.hip and kernel 0
+// (second order in time) a two-step pass in gr_wave2.hip.
 #include <hip/hip_runtime.h>
 
 #include "../../include/girih_gpu.h"
@@ -39,7 +39,6 @@ struct TbArgs {
   T *ZN;        // level TL
   const T *coef;
   long long cstride;
-  T c[5];
   long long sy, sz;
   int nx, ny, nz;  // interior; boxes are [R, n+R)
   int tiles_x, tiles_y, xcd_order;
@@ -50,7 +49,7 @@ struct TbArgs {
 
 template <typename T, int KID, int NT, int TXO, int TYO, int TL>
 __global__ __launch_bounds__(NT) void tb_kernel(const TbArgs<T> a) {
-  static_assert(KID == 1 || KID == 5, "first-order kernels only");
+  static_assert(KID == 5, "kernel 5 only");
   constexpr int R = kradius(KID);
   constexpr int Q = 2 * R + 1;
   constexpr int NC = kncoef(KID);
@@ -126,9 +125,6 @@ __global__ __launch_bounds__(NT) void tb_kernel(const TbArgs<T> a) {
   }
 
   auto planeY = [&](int k) { return a.Y + (long long)k * a.sz; };
-  T c[5];
-#pragma unroll
-  for (int r = 0; r < 5; ++r) c[r] = a.c[r];
 
   // queues: qv[t][j][slot], level t plane p in slot (p mod Q)
   T qv[TL][NCOL][Q];
@@ -208,9 +204,7 @@ __global__ __launch_bounds__(NT) void tb_kernel(const TbArgs<T> a) {
 #pragma unroll
               for (int d = -R; d <= R; ++d) zq[R + d] = zn(d);
               const T v0 = zq[R];
-              if constexpr (KID == 1) {  // stencils.c:73-78
-                out = c[0] * v0 + c[1] * (xn(1) + xn(-1)) + c[1] * (yn(1) + yn(-1)) + c[1] * (zq[2] + zq[0]);
-              } else {  // KID == 5, stencils.c:193-201; coefficients re-read by every level stay cached
+              {  // stencils.c:193-201; coefficients re-read by every level stay cached
                 const T *cp = a.coef + (long long)p * a.sz + off[j] / sizeof(T);
                 out = cp[0] * v0 + cp[a.cstride] * xn(-1) + cp[2 * a.cstride] * xn(1) +
                       cp[3 * a.cstride] * yn(-1) + cp[4 * a.cstride] * yn(1) + cp[5 * a.cstride] * zq[0] +
@@ -271,7 +265,7 @@ int tb_depths(int kid, int dtype, int *out, int max) {
   if (kid == 1 || kid == 5) {
     add(2);
     add(4);
-    add(6);
+    add(6);  // kernel 1: gr_fused1.hip, kernel 5: tb_kernel
   } else if (kid == 0) {
     add(2);  // gr_wave2.hip
   }
@@ -284,11 +278,7 @@ template <typename T>
 static hipError_t tb_dispatch(int kid, int tl, const TbArgs<T> &a, hipStream_t st) {
   constexpr bool F32 = std::is_same<T, float>::value;
   constexpr int TYS = F32 ? 32 : 16;  // 64 x TYS output tile, 512 lanes
-  if (kid == 1) {
-    if (tl == 2) return launch_tb<T, 1, 512, 64, TYS, 2>(a, st);
-    if (tl == 4) return launch_tb<T, 1, 512, 64, TYS, 4>(a, st);
-    if (tl == 6) return launch_tb<T, 1, 512, 64, 16, 6>(a, st);
-  } else if (kid == 5) {
+  if (kid == 5) {
     if (tl == 2) return launch_tb<T, 5, 512, 64, TYS, 2>(a, st);
     if (tl == 4) return launch_tb<T, 5, 512, 64, TYS, 4>(a, st);
     if (tl == 6) return launch_tb<T, 5, 512, 64, 16, 6>(a, st);
@@ -305,7 +295,6 @@ static hipError_t tb_pass_t(const Problem &P, int tl, const void *Y, void *ZN, c
   a.ZN = (T *)ZN;
   a.coef = (const T *)P.coef;
   a.cstride = P.field_elems;
-  for (int r = 0; r < 5; ++r) a.c[r] = (T)P.c[r];
   a.sy = P.pitch;
   a.sz = sz;
   a.nx = P.nx;
@@ -327,6 +316,7 @@ hipError_t tblock_pass(const Problem &P, int tl, const void *X, const void *Y, v
     if (tl != 2) return hipErrorInvalidValue;
     return wave2_pass(P, X, Y, ZO, ZN, src, src_idx, lc, st);
   }
+  if (P.kernel == 1) return jacobi_tb_pass(P, tl, Y, ZN, src, src_idx, lc, st);
   if (P.dtype == GR_F64) return tb_pass_t<double>(P, tl, Y, ZN, src, src_idx, lc, st);
   return tb_pass_t<float>(P, tl, Y, ZN, src, src_idx, lc, st);
 }

@@ -270,7 +270,7 @@ __global__ __launch_bounds__(64 * BX * BY) void wave2_kernel(const W2Args<T> a) 
       // ---- level 1: u^(n+1) at plane p1 over the tile plus R, into the LDS ring
       // (plane p1's ring entry was last read as plane p2 in the previous step)
       const bool zin1 = p1 >= R && p1 < a.nz + R;
-      const bool src1 = p1 == a.src_k;
+      const bool src1 = a.src_k >= 0 && p1 == a.src_k;  // src_k = -1: no source
       auto level1 = [&](auto q, unsigned off, int lw1, bool inner, T uo, T roc) {
         // neighbours at non-negative offsets from the cell's (-R, -R) corner, so that
         // every LDS read takes its offset as an immediate
@@ -295,7 +295,7 @@ __global__ __launch_bounds__(64 * BX * BY) void wave2_kernel(const W2Args<T> a) 
 
       // ---- level 2: u^(n+2) at plane p2 over the tile
       if (k >= zs + 2 * R) {
-        const bool src2 = p2 == a.src_k;
+        const bool src2 = a.src_k >= 0 && p2 == a.src_k;
         const auto wp = plane_rsrc(a.W + (long long)p2 * a.sz, pbytes);
         const auto yp = plane_rsrc(a.Y + (long long)p2 * a.sz, pbytes);
 #pragma unroll
