@@ -3,6 +3,10 @@
 The fused kernels keep girih's operand order, so FP64/FP32 results must stay
 bit-identical to the CPU oracle (and to single sweeps) for every depth, ragged
 size, step count and source -- including the final U1/U2 pair.
+
+The runs without a source are also the regression test of a fixed fault: the
+kernel-1 pass visits planes below 0 in its first z chunk, where the no-source
+marker (plane -1) once matched and a null source pointer was read.
 """
 import numpy as np
 import pytest
