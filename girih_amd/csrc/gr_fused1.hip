@@ -26,46 +26,12 @@
 // hold the same halo.
 #include <hip/hip_runtime.h>
 
-#include <utility>
-
 #include "../../include/girih_gpu.h"
 #include "gr_internal.hpp"
+#include "gr_plane.hpp"
 #include "gr_sweep.hpp"
 
 namespace gr {
-
-namespace {
-
-template <typename F, int... I>
-__device__ __forceinline__ void static_for_impl(F &&f, std::integer_sequence<int, I...>) {
-  (void)(f(std::integral_constant<int, I>{}) && ...);
-}
-template <int N, typename F>
-__device__ __forceinline__ void static_for(F &&f) {
-  static_for_impl(f, std::make_integer_sequence<int, N>{});
-}
-
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_rsrc(const void *base, unsigned bytes) {
-  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), 0, (int)bytes, 0x00020000);
-}
-template <typename T>
-__device__ __forceinline__ T bload(__amdgpu_buffer_rsrc_t r, unsigned off) {
-  if constexpr (sizeof(T) == 4)
-    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, (int)off, 0, 0));
-  else
-    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, 0));
-}
-template <bool NT, typename T>
-__device__ __forceinline__ void bstore(T v, __amdgpu_buffer_rsrc_t r, unsigned off) {
-  constexpr int aux = (NT && GR_STREAMING_HINTS) ? 2 : 0;
-  if constexpr (sizeof(T) == 4)
-    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, (int)off, 0, aux);
-  else
-    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(__attribute__((ext_vector_type(2))) unsigned, v), r,
-                                          (int)off, 0, aux);
-}
-
-}  // namespace
 
 template <typename T>
 struct J1Args {
