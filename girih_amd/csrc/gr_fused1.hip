@@ -187,15 +187,22 @@ static hipError_t launch_j1(J1Args<T> a, hipStream_t st) {
 }
 
 // Tiles: TL*2 LDS planes of (TX + 2TL) x (TY + 2TL) words must fit the 160 KB.
+static_assert(2 * 5 * (128 + 10) * (16 + 10) * 4 <= 160 * 1024, "FP32 depth-5 tile");
+static_assert(2 * 3 * (128 + 6) * (16 + 6) * 8 <= 160 * 1024, "FP64 depth-3 tile");
+static_assert(2 * 5 * (64 + 10) * (16 + 10) * 8 <= 160 * 1024, "FP64 depth-5 tile");
 template <typename T, bool STRM>
 static hipError_t j1_dispatch(int tl, const J1Args<T> &a, hipStream_t st) {
   if constexpr (sizeof(T) == 4) {
     if (tl == 2) return launch_j1<T, 2, 128, 32, 1024, STRM>(a, st);
+    if (tl == 3) return launch_j1<T, 3, 128, 16, 1024, STRM>(a, st);
     if (tl == 4) return launch_j1<T, 4, 128, 16, 1024, STRM>(a, st);
+    if (tl == 5) return launch_j1<T, 5, 128, 16, 1024, STRM>(a, st);
     if (tl == 6) return launch_j1<T, 6, 64, 16, 1024, STRM>(a, st);
   } else {
     if (tl == 2) return launch_j1<T, 2, 128, 16, 1024, STRM>(a, st);
+    if (tl == 3) return launch_j1<T, 3, 128, 16, 1024, STRM>(a, st);
     if (tl == 4) return launch_j1<T, 4, 64, 16, 1024, STRM>(a, st);
+    if (tl == 5) return launch_j1<T, 5, 64, 16, 1024, STRM>(a, st);
     if (tl == 6) return launch_j1<T, 6, 64, 8, 512, STRM>(a, st);
   }
   return hipErrorInvalidValue;

@@ -262,10 +262,12 @@ int tb_depths(int kid, int dtype, int *out, int max) {
     if (n < max) out[n++] = d;
   };
   (void)dtype;
-  if (kid == 1 || kid == 5) {
+  if (kid == 1) {  // gr_fused1.hip
+    for (int d = 2; d <= 6; ++d) add(d);
+  } else if (kid == 5) {  // tb_kernel
     add(2);
     add(4);
-    add(6);  // kernel 1: gr_fused1.hip, kernel 5: tb_kernel
+    add(6);
   } else if (kid == 0) {
     add(2);  // gr_wave2.hip
   }

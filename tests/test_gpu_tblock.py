@@ -21,7 +21,7 @@ pytestmark = pytest.mark.gpu
 SIZES = {1: [(70, 38, 20), (130, 9, 11), (64, 32, 8)], 4: [(72, 20, 17), (67, 33, 12)]}
 
 
-@pytest.mark.parametrize("kernel,depth", [(1, 2), (1, 4), (1, 6), (5, 2), (5, 4), (5, 6), (0, 2)])
+@pytest.mark.parametrize("kernel,depth", [(1, 2), (1, 3), (1, 4), (1, 5), (1, 6), (5, 2), (5, 4), (5, 6), (0, 2)])
 @pytest.mark.parametrize("prec", ["dp", "sp"])
 def test_fused_bitexact_vs_oracle(kernel, depth, prec):
     dt = np.float64 if prec == "dp" else np.float32
@@ -47,7 +47,7 @@ def test_fused_bitexact_vs_oracle(kernel, depth, prec):
             np.testing.assert_array_equal(U2[:, :, :nnx], v)
 
 
-@pytest.mark.parametrize("kernel,depth", [(1, 2), (1, 6), (5, 4), (0, 2)])
+@pytest.mark.parametrize("kernel,depth", [(1, 2), (1, 3), (1, 5), (1, 6), (5, 4), (0, 2)])
 @pytest.mark.parametrize("zchunk", [1, 5, 8])
 def test_fused_zchunks_bitexact_vs_oracle(kernel, depth, zchunk):
     """Workgroups that own a z chunk recompute the chunk's z ghost planes; the
@@ -238,7 +238,8 @@ def test_autotune_keeps_fields_and_may_pick_fused():
 
 
 def test_set_config_rejects_unavailable_depth():
-    with gpu.Stepper(1, 16, 16, 16) as st:
-        with pytest.raises(gpu.GirihError) as ei:
-            st.set_config(t_block=3)
-        assert ei.value.code == gpu.GR_EUNSUPPORTED
+    for kernel, depth in ((1, 7), (5, 3), (0, 4), (2, 2)):
+        with gpu.Stepper(kernel, 16, 16, 16) as st:
+            with pytest.raises(gpu.GirihError) as ei:
+                st.set_config(t_block=depth)
+            assert ei.value.code == gpu.GR_EUNSUPPORTED
