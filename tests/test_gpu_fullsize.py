@@ -7,7 +7,10 @@
   a field symmetric under every axis reflection stays symmetric bit for bit
   (floating-point addition is commutative, so mirrored cells evaluate the same
   sums), which the oracle cannot check at this size in reasonable time;
-* 1^3 .. 3^3 grids for every kernel: bit-identical to the oracle.
+* 1^3 .. 3^3 grids for every kernel: bit-identical to the oracle;
+* fused passes at the benchmark sizes (7-point Jacobi 512^3 FP32 depth 4 and FP64
+  depth 3, 25-point wave 1024^3 FP32 two-step passes): bit-identical to single
+  sweeps, compared on the device.
 """
 import numpy as np
 import pytest
@@ -93,3 +96,26 @@ def test_smallest_grids_bitexact(kernel):
             orc.sweep(kernel, coef, u, v, roc2)
             np.testing.assert_array_equal(U1[:, :, :u.shape[2]], u, err_msg="%s %d" % (prec, n))
             np.testing.assert_array_equal(U2[:, :, :u.shape[2]], v, err_msg="%s %d" % (prec, n))
+
+
+@pytest.mark.parametrize("kernel,n,prec,depth", [(1, 512, "sp", 4), (1, 512, "dp", 3), (0, 1024, "sp", 2)])
+def test_fused_equals_sweeps_at_benchmark_size(kernel, n, prec, depth):
+    dtype = gpu.GR_F32 if prec == "sp" else gpu.GR_F64
+    if kernel == 0:
+        kw = dict(flags=gpu.GR_FAST | gpu.GR_ROC2_SCALAR, roc2=0.16, coef=physics.fd8_coef())
+    else:
+        kw = dict(coef=[0.4, 0.1])
+    napply = 2 * depth + (0 if kernel == 0 else 1)  # first-order kernels end on a single sweep
+    print("fused at size", kernel, n, prec, depth, flush=True)
+    with gpu.Stepper(kernel, n, n, n, dtype=dtype, **kw) as a, gpu.Stepper(kernel, n, n, n, dtype=dtype, **kw) as b:
+        for st in (a, b):
+            st.init_fields(1, 4321)  # interior U(0,1), U1 = U2, zero halos
+        a.set_config(13, 64, 1)
+        b.set_config(13, 256, depth)
+        a.step(napply)
+        b.step(napply)
+        assert b.stats()["last_launches"] < napply
+        for f in (0, 1):
+            c = b.compare(f, a, f)
+            assert c["max_ref"] > 0
+            assert c["max_diff"] == 0.0 and c["sq_diff"] == 0.0, (f, c)
