@@ -46,4 +46,20 @@ __device__ __forceinline__ void bstore(T v, __amdgpu_buffer_rsrc_t r, unsigned o
                                           (int)off, 0, aux);
 }
 
+// A coefficient or roc2 word of storage type C at element offset e of a plane
+// resource, widened to T (bf16 widens exactly: its bits move to the top half).
+// NT: non-temporal load (the same cache-policy bit as bstore).
+template <typename T, typename C, bool NT>
+__device__ __forceinline__ T bload_coef(__amdgpu_buffer_rsrc_t r, unsigned e) {
+  constexpr int aux = (NT && GR_STREAMING_HINTS) ? 2 : 0;
+  if constexpr (sizeof(C) == 2) {
+    const unsigned short b = __builtin_amdgcn_raw_buffer_load_b16(r, (int)(e * 2u), 0, aux);
+    return static_cast<T>(__uint_as_float((unsigned)b << 16));
+  } else if constexpr (sizeof(C) == 4) {
+    return static_cast<T>(__builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)(e * 4u), 0, aux)));
+  } else {
+    return static_cast<T>(__builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)(e * 8u), 0, aux)));
+  }
+}
+
 }  // namespace gr
