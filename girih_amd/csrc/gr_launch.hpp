@@ -35,8 +35,12 @@ inline hipError_t launch_cfg(const SweepArgs<T, C> &a0, const LaunchCfg &lc, hip
   a.nzc = (bz + a.zchunk - 1) / a.zchunk;
   a.xcd_order = lc.xcd_order;
   const int nb = a.tiles_x * a.tiles_y * a.nzc;
-  hipLaunchKernelGGL((sweep_kernel<T, C, KID, BY, RY, FAST, ROCS, MINW, RING, kFastND, BX, NT>), dim3(nb), dim3(TX, BY),
-                     0, st, a);
+  // Buffer-resource plane accesses where they measured faster: FP32 kernel 0 with a
+  // roc2 field (bf16 roc2 at 1024^3: 307 -> 353 GLUP/s).  Scalar-roc2 FP32 (-2%) and
+  // FP64 (-11%) sweeps keep SGPR-base global loads.
+  constexpr bool BUF = KID == 0 && !ROCS && std::is_same<T, float>::value;
+  hipLaunchKernelGGL((sweep_kernel<T, C, KID, BY, RY, FAST, ROCS, MINW, RING, kFastND, BX, NT, BUF>), dim3(nb),
+                     dim3(TX, BY), 0, st, a);
   return hipGetLastError();
 }
 

@@ -28,12 +28,14 @@ __device__ __forceinline__ void static_for(F &&f) {
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_rsrc(const void *base, unsigned bytes) {
   return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), 0, (int)bytes, 0x00020000);
 }
-template <typename T>
+// NT: non-temporal load (the same cache-policy bit as bstore).
+template <typename T, bool NT = false>
 __device__ __forceinline__ T bload(__amdgpu_buffer_rsrc_t r, unsigned off) {
+  constexpr int aux = (NT && GR_STREAMING_HINTS) ? 2 : 0;
   if constexpr (sizeof(T) == 4)
-    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, (int)off, 0, 0));
+    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, (int)off, 0, aux));
   else
-    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, 0));
+    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, aux));
 }
 // NT: non-temporal store (the CDNA cache-policy bit NT = 2).
 template <bool NT, typename T>

@@ -233,8 +233,10 @@ struct Tile {
   static constexpr int RY = (KID == 4) ? 2 : 4;
 };
 
+// BUF: every plane access through a buffer resource (scalar plane base, 32-bit lane
+// offset) instead of global loads; measured per case, see the launch code.
 template <typename T, typename C, int KID, int BY, int RY, bool FAST, bool ROCS, bool RING, int ND, int BX = 1,
-          bool STRM = false>
+          bool STRM = false, bool BUF = false>
 struct SweepTile;
 
 // MINW > 0 asks the compiler for at least MINW waves per SIMD (caps VGPRs at 512/MINW).
@@ -243,10 +245,10 @@ struct SweepTile;
 // the neighbouring tiles fetch it too and it is still in L2; otherwise the LDS is
 // double buffered and the halo is fetched one plane ahead.
 template <typename T, typename C, int KID, int BY, int RY, bool FAST, bool ROCS, int MINW, bool RING, int ND = kFastND,
-          int BX = 1, bool STRM = false>
+          int BX = 1, bool STRM = false, bool BUF = false>
 __global__ __launch_bounds__(64 * BX * BY) __attribute__((amdgpu_waves_per_eu(MINW > 0 ? MINW : 1)))
 void sweep_kernel(const SweepArgs<T, C> a) {
-  using ST = SweepTile<T, C, KID, BY, RY, FAST, ROCS, RING, ND, BX, STRM>;
+  using ST = SweepTile<T, C, KID, BY, RY, FAST, ROCS, RING, ND, BX, STRM, BUF>;
   __shared__ typename ST::Lds lds;
   const int nb = a.tiles_x * a.tiles_y * a.nzc;
   const int lb = a.xcd_order ? xcd_remap(blockIdx.x, nb) : (int)blockIdx.x;
@@ -258,7 +260,8 @@ void sweep_kernel(const SweepArgs<T, C> a) {
 
 // One tile of one sweep: the z-streaming body shared by the single-grid kernel
 // and the batched persistent kernel.
-template <typename T, typename C, int KID, int BY, int RY, bool FAST, bool ROCS, bool RING, int ND, int BX, bool STRM>
+template <typename T, typename C, int KID, int BY, int RY, bool FAST, bool ROCS, bool RING, int ND, int BX, bool STRM,
+          bool BUF>
 struct SweepTile {
   static constexpr int R = kradius(KID);
   static constexpr int TX = 64 * BX, TY = BY * RY;
@@ -268,8 +271,9 @@ struct SweepTile {
   __device__ static __forceinline__ void run(const SweepArgs<T, C> &a, int tile_x, int tile_y, int zc, Lds &lds);
 };
 
-template <typename T, typename C, int KID, int BY, int RY, bool FAST, bool ROCS, bool RING, int ND, int BX, bool STRM>
-__device__ __forceinline__ void SweepTile<T, C, KID, BY, RY, FAST, ROCS, RING, ND, BX, STRM>::run(const SweepArgs<T, C> &a,
+template <typename T, typename C, int KID, int BY, int RY, bool FAST, bool ROCS, bool RING, int ND, int BX, bool STRM,
+          bool BUF>
+__device__ __forceinline__ void SweepTile<T, C, KID, BY, RY, FAST, ROCS, RING, ND, BX, STRM, BUF>::run(const SweepArgs<T, C> &a,
                                                                                    int tile_x, int tile_y, int zc,
                                                                                    Lds &lds) {
   constexpr int Q = 2 * R + 1;
@@ -316,30 +320,50 @@ __device__ __forceinline__ void SweepTile<T, C, KID, BY, RY, FAST, ROCS, RING, N
   }
   auto plane = [&](const T *base, int k) { return base + (long long)k * g.sz; };
   const int zlast = g.ze + R - 1;  // last plane any load may touch
+  // plane accesses: buffer resources (BUF) or SGPR-base global loads
+  const unsigned pbytes = (unsigned)g.sz * (unsigned)sizeof(T);
+  auto pl = [&](const T *base, int k) {
+    if constexpr (BUF)
+      return plane_rsrc(base + (long long)k * g.sz, pbytes);
+    else
+      return plane(base, k);
+  };
+  auto ld = [&](auto p, unsigned o) -> T {
+    if constexpr (BUF)
+      return bload<T>(p, o);
+    else
+      return ldb<T>(p, o);
+  };
+  auto ld_strm = [&](auto p, unsigned o) -> T {
+    if constexpr (BUF)
+      return bload<T, STRM>(p, o);
+    else
+      return ldb_stream<T, STRM>(p, o);
+  };
 
   // z queue of the tile interior: slot i holds plane k-R+((i - p) mod Q) at phase p;
   // the z loop is unrolled by Q so the rotation is static register renaming.
   T q[RY][Q];
 #pragma unroll
   for (int i = 0; i < Q; ++i) {
-    const T *vp = plane(a.v, z0 - R + i);
+    const auto vp = pl(a.v, z0 - R + i);
 #pragma unroll
-    for (int m = 0; m < RY; ++m) q[m][i] = ldb<T>(vp, off[m]);
+    for (int m = 0; m < RY; ++m) q[m][i] = ld(vp, off[m]);
   }
   T hv[HS];
   if constexpr (RING) {
     for (int j = 0; j < R; ++j) {  // halo rings of planes z0 .. z0+R-1 straight into their slots
-      const T *vp = plane(a.v, z0 + j);
+      const auto vp = pl(a.v, z0 + j);
       T *Ls = &lds[(z0 + j) % S][0][0];
 #pragma unroll
       for (int s = 0; s < HS; ++s)
-        if (hw[s]) Ls[hl[s]] = ldb<T>(vp, hoff[s]);
+        if (hw[s]) Ls[hl[s]] = ld(vp, hoff[s]);
     }
   }
   {
-    const T *vp = plane(a.v, min(RING ? z0 + R : z0, zlast));
+    const auto vp = pl(a.v, min(RING ? z0 + R : z0, zlast));
 #pragma unroll
-    for (int s = 0; s < HS; ++s) hv[s] = ldb<T>(vp, hoff[s]);
+    for (int s = 0; s < HS; ++s) hv[s] = ld(vp, hoff[s]);
   }
   T c[5];
 #pragma unroll
@@ -365,7 +389,7 @@ __device__ __forceinline__ void SweepTile<T, C, KID, BY, RY, FAST, ROCS, RING, N
       T uo[RY], r2[RY];
       T cf[RY][NC > 0 ? NC : 1];
       {
-        const T *up = plane(a.uold ? a.uold : a.u, k);
+        const auto up = pl(a.uold ? a.uold : a.u, k);
         const long long pk = (long long)k * g.sz;
         // The roc2 plane is read through a buffer resource (scalar plane base, 32-bit
         // lane offset): 11% faster for kernel 0 with a bf16 roc2 field at 1024^3.  The
@@ -375,7 +399,7 @@ __device__ __forceinline__ void SweepTile<T, C, KID, BY, RY, FAST, ROCS, RING, N
 #pragma unroll
         for (int m = 0; m < RY; ++m) {
           if constexpr (KID == 0) {
-            uo[m] = ldb_stream<T, STRM>(up, off[m]);
+            uo[m] = ld_strm(up, off[m]);
             if constexpr (!ROCS)
               r2[m] = bload_coef<T, C, STRM>(plane_rsrc(a.roc2 + pk, cbytes), off[m] / (unsigned)sizeof(T));
           }
@@ -387,12 +411,12 @@ __device__ __forceinline__ void SweepTile<T, C, KID, BY, RY, FAST, ROCS, RING, N
       // plane k+R+1 into the slot freed by plane k-R, halo ring of plane k+1
       T nq[RY];
       {
-        const T *vp = plane(a.v, min(k + 1 + R, zlast));
+        const auto vp = pl(a.v, min(k + 1 + R, zlast));
 #pragma unroll
-        for (int m = 0; m < RY; ++m) nq[m] = ldb<T>(vp, off[m]);
-        const T *hp = plane(a.v, min(RING ? k + 1 + R : k + 1, zlast));
+        for (int m = 0; m < RY; ++m) nq[m] = ld(vp, off[m]);
+        const auto hp = pl(a.v, min(RING ? k + 1 + R : k + 1, zlast));
 #pragma unroll
-        for (int s = 0; s < HS; ++s) hv[s] = ldb<T>(hp, hoff[s]);
+        for (int s = 0; s < HS; ++s) hv[s] = ld(hp, hoff[s]);
       }
 
       __syncthreads();
@@ -435,7 +459,12 @@ __device__ __forceinline__ void SweepTile<T, C, KID, BY, RY, FAST, ROCS, RING, N
                 cf[m][4] * yn(1) + cf[m][5] * zq[0] + cf[m][6] * zq[2];
         }
         if (src_plane && off[m] == (unsigned)a.src_off * (unsigned)sizeof(T)) out = out + *a.src;
-        if (yout[m]) stb_stream<T, STRM>(uk, off[m], out);
+        if (yout[m]) {
+          if constexpr (BUF)
+            bstore<STRM>(out, plane_rsrc(uk, pbytes), off[m]);
+          else
+            stb_stream<T, STRM>(uk, off[m], out);
+        }
       }
 #pragma unroll
       for (int m = 0; m < RY; ++m) q[m][p] = nq[m];
