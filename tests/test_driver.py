@@ -57,6 +57,8 @@ def test_invalid_argument_prints_usage():
      "--thread-group-size", 1],
     ["--target-ts", 2, "--target-kernel", 1, "--nx", 70, "--ny", 32, "--nz", 24, "--nt", 9, "--t-dim", 3],
     ["--target-ts", 3, "--target-kernel", 5, "--nx", 33, "--ny", 17, "--nz", 12, "--nt", 7],
+    ["--target-ts", 3, "--target-kernel", 1, "--nx", 70, "--ny", 40, "--nz", 30, "--nt", 11],
+    ["--target-ts", 3, "--target-kernel", 0, "--nx", 72, "--ny", 40, "--nz", 36, "--nt", 8],
     ["--target-ts", 1, "--target-kernel", 4, "--nx", 20, "--ny", 12, "--nz", 10, "--nt", 4],
     ["--target-ts", 0, "--target-kernel", 2, "--nx", 20, "--ny", 12, "--nz", 10, "--nt", 5],
     ["--target-ts", 0, "--target-kernel", 3, "--nx", 20, "--ny", 12, "--nz", 10, "--nt", 5],
