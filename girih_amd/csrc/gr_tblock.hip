@@ -52,7 +52,6 @@ __global__ __launch_bounds__(NT) void tb_kernel(const TbArgs<T> a) {
   static_assert(KID == 5, "kernel 5 only");
   constexpr int R = kradius(KID);
   constexpr int Q = 2 * R + 1;
-  constexpr int NC = kncoef(KID);
   constexpr int W0 = TXO + 2 * R * TL, H0 = TYO + 2 * R * TL;          // level-0 region
   constexpr int W1 = TXO + 2 * R * (TL - 1), H1 = TYO + 2 * R * (TL - 1);  // level-1 region
   constexpr int NCOL = (W1 * H1 + NT - 1) / NT;                            // columns per lane
