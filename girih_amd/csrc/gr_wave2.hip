@@ -67,7 +67,7 @@ __global__ __launch_bounds__(64 * BX * BY) void wave2_kernel(const W2Args<T> a) 
   constexpr int W0 = TX + 4 * R, H0 = TY + 4 * R;  // level-0 region (tile + 2R)
   // LDS row pitch: a multiple of 64 words where it fits, so that the y neighbours
   // of a cell are read pairwise from one base (ds_read2st64) with no address adds
-  constexpr int LP = (sizeof(T) == 4) ? (W0 + 63) / 64 * 64 : W0;
+  constexpr int LP = (sizeof(T) == 4 && BX >= 2) ? (W0 + 63) / 64 * 64 : W0;
   constexpr int NE = 2 * R * H1;                   // level-1 cells left and right of the tile columns
   constexpr int NEW = (NE + 63) / 64;              // wavefronts that own them
   static_assert(NEW * 64 <= NT, "margin columns need more lanes than the workgroup has");
@@ -299,12 +299,13 @@ static hipError_t launch_w2(W2Args<T> a, hipStream_t st) {
   return hipGetLastError();
 }
 
-// Output tiles: FP32 128 x 16 (1024 lanes, 145 KB LDS); FP64 64 x 16 (512 lanes,
-// 124 KB LDS).  A 128 x 8 FP64 tile of 1024 lanes (four wavefronts per SIMD) spills
-// its queues out of the 128 registers that allows and measured 25% slower.
+// Output tiles of 64 x 16 cells and 512 lanes (FP32: 64 KB LDS, two workgroups per
+// CU; FP64: 128 KB).  In FP32 the 128 x 16 tile of 1024 lanes (one workgroup per CU,
+// the 192-word LDS pitch) measured 2-4% slower; a 128 x 8 FP64 tile of 1024 lanes
+// spilled its queues and measured 25% slower.
 template <typename T, bool FAST>
 static hipError_t w2_dispatch(bool rocs, bool strm, const W2Args<T> &a, hipStream_t st) {
-  constexpr int BX = sizeof(T) == 4 ? 2 : 1;
+  constexpr int BX = 1;
   if (rocs) {
     if (strm) return launch_w2<T, FAST, true, true, BX, 8, 2>(a, st);
     return launch_w2<T, FAST, true, false, BX, 8, 2>(a, st);
