@@ -18,13 +18,16 @@
 //     the 2R x (TY+2R) margin columns left and right of the tile belong to the
 //     first few wavefronts as one more slot;
 //   * each lane keeps a register queue of 2R+1 planes of u^n for its columns
-//     (level 1's z neighbours) and, for its output columns, of u^(n+1) (level
-//     2's); the centre plane of u^n (level 1's xy neighbours, over the tile plus
+//     (level 1's z neighbours; the margin-column slot keeps this queue in LDS)
+//     and, for its output columns, of u^(n+1) (level 2's); the centre plane of u^n (level 1's xy neighbours, over the tile plus
 //     2R) is staged in a double-buffered LDS plane, and level 1 writes u^(n+1)
 //     straight into an LDS ring of R+1 planes, from which level 2 reads the xy
 //     neighbours of its centre plane R steps later;
-//   * every global load is issued one plane ahead and clamped into the field,
-//     so loads need no predicate.
+//   * every plane load goes through a buffer resource (scalar plane base, 32-bit
+//     lane offset), is issued one plane ahead and is clamped into the field, so
+//     loads need no predicate;
+//   * the plane loop is unrolled by construction (static_for), so every queue
+//     access is a fixed register.
 //
 // The arithmetic of each level is the single sweep's (Op<T, 0, FAST>), so a
 // pass is bit-identical to two sweeps.  Boundary cells (girih's halo, never
