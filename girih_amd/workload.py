@@ -2,6 +2,7 @@
 
     python -m girih_amd.workload --kernel 5 --n 512 --dtype f32 --config 0,256,0,6 --steps 12
     python -m girih_amd.workload --kernel 0 --n 1024 --dtype f64 --config 13,0,0,1 --steps 10
+    python -m girih_amd.workload --batch --config 1,64 --steps 20
 
 The fields start from the seeded uniform state (--init random: interior U(0,1);
 coefficient arrays centre U[0.3,0.5), neighbours U[0,0.1); kernel 0: the 1024^3
@@ -10,6 +11,11 @@ wave set-up, v = 3000 m/s, h = 10 m, CFL 0.4, scalar roc2), from zero fields
 VARIANT,ZCHUNK,XCD_ORDER,T_BLOCK as in gr_set_config / gr_set_xcd_order; with no
 --config the autotuner picks one.  One line of JSON: the configuration, the
 applications per launch and the GLUP/s of the timed steps (HIP events).
+
+--batch runs the 16-problem wave batch instead (kernel 0, FP32 GR_FAST, bf16 roc2
+from 8-layer velocity models, dims uniform in 192..512 step 16 from seed 2024, a
+Ricker source at each centre), with --config VARIANT,ZCHUNK of gr_batch_set_config
+or the batch autotuner, one persistent launch per step replayed from a HIP graph.
 """
 import argparse
 import json
@@ -17,16 +23,48 @@ import json
 from . import gpu, physics
 
 
+def batch_main(a):
+    import numpy as np
+    rng = np.random.default_rng(2024)
+    sizes = [tuple(int(x) for x in rng.choice(np.arange(192, 513, 16), 3)) for _ in range(16)]
+    h = 10.0
+    dt = physics.cfl_dt(4500.0 * 1.05, h, 0.4)
+    src = physics.roc2_const(3000.0, dt, h) * physics.ricker(15.0, dt, a.steps + 16)
+    cells = sum(x * y * z for x, y, z in sizes)
+    with gpu.Batch(sizes, flags=gpu.GR_FAST | gpu.GR_COEF_BF16, coef=physics.fd8_coef()) as b:
+        for i, p in enumerate(b.problems):
+            p.init_fields(2)
+            p.init_roc2_model(gpu.vmodel(2024 + i, dt / h))
+            nx, ny, nz = sizes[i]
+            p.set_source(4 + nx // 2, 4 + ny // 2, 4 + nz // 2, src)
+        if a.config:
+            var, zc = (int(x) for x in a.config.split(","))
+            b.set_config(var, zc)
+        else:
+            b.autotune(4, use_graph=True)
+        b.step(2, use_graph=True)
+        b.step(a.steps, use_graph=True)
+        s = b.stats()
+        print(json.dumps({"batch": len(sizes), "cells": cells, "config": "%d,%d" % (s["variant"], s["zchunk"]),
+                          "tile": [s["tile_x"], s["tile_y"]], "steps": a.steps,
+                          "glups": cells * a.steps / s["last_ms"] / 1e6}))
+
+
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--kernel", type=int, required=True)
-    ap.add_argument("--n", type=int, required=True)
+    ap.add_argument("--batch", action="store_true", help="the 16-problem bf16 wave batch")
+    ap.add_argument("--kernel", type=int)
+    ap.add_argument("--n", type=int)
     ap.add_argument("--dtype", choices=["f32", "f64"], default="f32")
     ap.add_argument("--bf16", action="store_true", help="coefficient arrays / roc2 stored as bf16")
     ap.add_argument("--config", default=None)
     ap.add_argument("--steps", type=int, default=12)
     ap.add_argument("--init", choices=["random", "zero", "girih"], default="random")
     a = ap.parse_args()
+    if a.batch:
+        return batch_main(a)
+    if a.kernel is None or a.n is None:
+        ap.error("--kernel and --n are required (or --batch)")
     dtype = gpu.GR_F32 if a.dtype == "f32" else gpu.GR_F64
     flags = gpu.GR_COEF_BF16 if a.bf16 else 0
     coef = None
