@@ -56,20 +56,24 @@ def test_fused_zchunks_bitexact_vs_oracle(kernel, depth, zchunk):
     napply = 2 * depth + (0 if kernel == 0 else 1)
     print("fused zchunk", kernel, depth, zchunk, flush=True)
     u, v, roc2, coef = _random_problem(kernel, nx, ny, nz, np.float64, 500 + kernel)
+    outs = []
     with gpu.Stepper(kernel, nx, ny, nz, dtype=gpu.GR_F64, coef=_consts(coef, kernel)) as st:
-        st.upload(_pad(u), _pad(v), _pad(roc2) if roc2 is not None else None, _pad_coef(coef, kernel))
-        st.set_config(zchunk=zchunk, t_block=depth)
-        st.step(napply)
-        assert st.stats()["last_launches"] < napply
-        U1, U2 = st.download(_pad(u).shape)
+        for order in (0, 1):  # plain and XCD-contiguous workgroup order
+            st.upload(_pad(u), _pad(v), _pad(roc2) if roc2 is not None else None, _pad_coef(coef, kernel))
+            st.set_config(zchunk=zchunk, t_block=depth)
+            st.set_xcd_order(order)
+            st.step(napply)
+            assert st.stats()["last_launches"] < napply
+            outs.append(st.download(_pad(u).shape))
     for a in range(napply):
         if a % 2 == 0:
             orc.sweep(kernel, coef, u, v, roc2)
         else:
             orc.sweep(kernel, coef, v, u, roc2)
     nnx = u.shape[2]
-    np.testing.assert_array_equal(U1[:, :, :nnx], u)
-    np.testing.assert_array_equal(U2[:, :, :nnx], v)
+    for order, (U1, U2) in enumerate(outs):
+        np.testing.assert_array_equal(U1[:, :, :nnx], u, err_msg="order %d" % order)
+        np.testing.assert_array_equal(U2[:, :, :nnx], v, err_msg="order %d" % order)
 
 
 def test_fused_wave_with_source_fp64():
