@@ -93,9 +93,22 @@ def cpu_baseline(n, budget_nt):
     if r.returncode != 0 or not m:
         return {"value": None, "unit": "GLUP/s", "cores": cores, "kind": "reference",
                 "sample": "failed: rc %d %s" % (r.returncode, (r.stderr or r.stdout)[-200:])}
-    return {"value": float(m.group(1)) / 1e3, "unit": "GLUP/s", "cores": cores, "kind": "reference",
-            "sample": "girih mwd_kernel (DP, OpenMP 1WD diamond, t_dim 1, kernel 0 with girih's own "
-                      "coefficients) %d^3 x nt %d, n-tests 1, %.1f s wall incl. init" % (n, nt, wall)}
+    out = {"value": float(m.group(1)) / 1e3, "unit": "GLUP/s", "cores": cores, "kind": "reference",
+           "sample": "girih mwd_kernel (DP, OpenMP 1WD diamond, t_dim 1, kernel 0 with girih's own "
+                     "coefficients) %d^3 x nt %d, n-tests 1, %.1f s wall incl. init" % (n, nt, wall)}
+    # girih's spatially blocked stepper (ts 0) on the same cores, reported beside the diamond stepper
+    sb_nt = 6
+    cmd_sb = [exe, "--nx", str(n), "--ny", str(n), "--nz", str(n), "--nt", str(sb_nt), "--target-ts", "0",
+              "--target-kernel", "0", "--thread-group-size", "1", "--thx", "1", "--thy", "1", "--thz", "1",
+              "--t-dim", "1", "--num-wavefronts", "1", "--n-tests", "1", "--verbose", "0"]
+    try:
+        r = subprocess.run(cmd_sb, env=env, capture_output=True, text=True, timeout=300)
+        m = re.search(r"RANK0 GStencil/s MEDIAN:\s*([0-9.eE+-]+)", r.stdout)
+        out["sb_value"] = float(m.group(1)) if (r.returncode == 0 and m) else None
+    except (OSError, subprocess.SubprocessError):
+        out["sb_value"] = None
+    out["sb_sample"] = "girih mwd_kernel spatial blocking (ts 0, DP) %d^3 x nt %d, n-tests 1" % (n, sb_nt)
+    return out
 
 
 def cpu_baseline_port():
