@@ -186,10 +186,8 @@ static hipError_t launch_j1(J1Args<T> a, hipStream_t st) {
   return hipGetLastError();
 }
 
-// Tiles: TL*2 LDS planes of (TX + 2TL) x (TY + 2TL) words must fit the 160 KB.
-static_assert(2 * 5 * (128 + 10) * (16 + 10) * 4 <= 160 * 1024, "FP32 depth-5 tile");
-static_assert(2 * 3 * (128 + 6) * (16 + 6) * 8 <= 160 * 1024, "FP64 depth-3 tile");
-static_assert(2 * 5 * (64 + 10) * (16 + 10) * 8 <= 160 * 1024, "FP64 depth-5 tile");
+// Tiles: TL*2 LDS planes of (TX + 2TL) x (TY + 2TL) words must fit the 160 KB (the
+// compiler rejects a kernel whose LDS exceeds it).
 template <typename T, bool STRM>
 static hipError_t j1_dispatch(int tl, const J1Args<T> &a, hipStream_t st) {
   if constexpr (sizeof(T) == 4) {
