@@ -10,7 +10,8 @@ wave set-up, v = 3000 m/s, h = 10 m, CFL 0.4, scalar roc2), from zero fields
 (--init zero) or from girih's closed-form state (--init girih).  --config is
 VARIANT,ZCHUNK,XCD_ORDER,T_BLOCK as in gr_set_config / gr_set_xcd_order; with no
 --config the autotuner picks one.  One line of JSON: the configuration, the
-applications per launch and the GLUP/s of the timed steps (HIP events).
+applications per launch and the GLUP/s of the timed steps (HIP events), which follow
+one untimed pass (--warmup 0 drops it).
 
 --batch runs the 16-problem wave batch instead (kernel 0, FP32 GR_FAST, bf16 roc2
 from 8-layer velocity models, dims uniform in 192..512 step 16 from seed 2024, a
@@ -60,6 +61,8 @@ def main():
     ap.add_argument("--config", default=None)
     ap.add_argument("--steps", type=int, default=12)
     ap.add_argument("--init", choices=["random", "zero", "girih"], default="random")
+    ap.add_argument("--warmup", type=int, choices=[0, 1], default=1,
+                    help="one untimed pass first, so the timed steps exclude the first launch's code-object load")
     a = ap.parse_args()
     if a.batch:
         return batch_main(a)
@@ -87,6 +90,8 @@ def main():
     tb = s["t_block"]
     # whole fused passes (first-order kernels end a step count with one single sweep)
     napply = a.steps * tb + (0 if a.kernel == 0 or tb == 1 else 1)
+    if a.warmup:
+        st.step(tb if a.kernel == 0 or tb == 1 else tb + 1)  # loads the code objects
     st.step(napply)
     s = st.stats()
     print(json.dumps({"kernel": a.kernel, "n": a.n, "dtype": a.dtype, "bf16": a.bf16, "init": a.init,
