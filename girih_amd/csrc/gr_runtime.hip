@@ -471,7 +471,7 @@ static int fused_depth_ok(gr_handle *h, int tl, bool *ok) {
   bool have = false;
   for (int i = 0; i < nd; ++i) have |= depths[i] == tl;
   if (!have) return GR_OK;
-  if ((h->P.flags & GR_COEF_BF16) != 0) return GR_OK;
+  if ((h->P.flags & GR_COEF_BF16) != 0 && h->P.kernel != 5) return GR_OK;  // bf16 fused: kernel 5 only
   int rc = check_halo_equal(h);
   if (rc) return rc;
   *ok = h->halo_equal == 1;

@@ -22,10 +22,11 @@
 // Boundary cells (girih's halo, never updated) keep their input value at every
 // level; the host enables this path only when U1 and U2 hold the same halo.
 //
-// This kernel serves kernel 5 (seven coefficient arrays): level t = f(level t-1),
-// input Y, output ZN.  Kernel 1 has a leaner pass in gr_fused1.hip and kernel 0
+// This kernel serves kernel 5 (seven coefficient arrays, FP64, FP32 or bf16-stored):
+// level t = f(level t-1), input Y, output ZN.  Kernel 1 has a leaner pass in gr_fused1.hip and kernel 0
 // (second order in time) a two-step pass in gr_wave2.hip.
 #include <hip/hip_runtime.h>
+#include <hip/hip_bf16.h>
 
 #include "../../include/girih_gpu.h"
 #include "gr_internal.hpp"
@@ -33,11 +34,12 @@
 
 namespace gr {
 
-template <typename T>
+// C: storage type of the coefficient arrays (T, or bf16 widened exactly to FP32)
+template <typename T, typename C>
 struct TbArgs {
   const T *Y;   // level 0
   T *ZN;        // level TL
-  const T *coef;
+  const C *coef;
   long long cstride;
   long long sy, sz;
   int nx, ny, nz;  // interior; boxes are [R, n+R)
@@ -47,8 +49,8 @@ struct TbArgs {
   int src_k, src_off;
 };
 
-template <typename T, int KID, int NT, int TXO, int TYO, int TL>
-__global__ __launch_bounds__(NT) void tb_kernel(const TbArgs<T> a) {
+template <typename T, typename C, int KID, int NT, int TXO, int TYO, int TL>
+__global__ __launch_bounds__(NT) void tb_kernel(const TbArgs<T, C> a) {
   static_assert(KID == 5, "kernel 5 only");
   constexpr int R = kradius(KID);
   constexpr int Q = 2 * R + 1;
@@ -204,10 +206,10 @@ __global__ __launch_bounds__(NT) void tb_kernel(const TbArgs<T> a) {
               for (int d = -R; d <= R; ++d) zq[R + d] = zn(d);
               const T v0 = zq[R];
               {  // stencils.c:193-201; coefficients re-read by every level stay cached
-                const T *cp = a.coef + (long long)p * a.sz + off[j] / sizeof(T);
-                out = cp[0] * v0 + cp[a.cstride] * xn(-1) + cp[2 * a.cstride] * xn(1) +
-                      cp[3 * a.cstride] * yn(-1) + cp[4 * a.cstride] * yn(1) + cp[5 * a.cstride] * zq[0] +
-                      cp[6 * a.cstride] * zq[2];
+                const C *cp = a.coef + (long long)p * a.sz + off[j] / sizeof(T);
+                auto cf = [&](int m) { return ld_coef<T, C, false>(cp + m * a.cstride); };
+                out = cf(0) * v0 + cf(1) * xn(-1) + cf(2) * xn(1) + cf(3) * yn(-1) + cf(4) * yn(1) +
+                      cf(5) * zq[0] + cf(6) * zq[2];
               }
               if (p == a.src_k && off[j] == (unsigned)a.src_off * (unsigned)sizeof(T)) out = out + a.src[t - 1];
             } else {
@@ -243,14 +245,14 @@ __global__ __launch_bounds__(NT) void tb_kernel(const TbArgs<T> a) {
   }
 }
 
-template <typename T, int KID, int NT, int TXO, int TYO, int TL>
-static hipError_t launch_tb(const TbArgs<T> &a0, hipStream_t st) {
-  TbArgs<T> a = a0;
+template <typename T, typename C, int KID, int NT, int TXO, int TYO, int TL>
+static hipError_t launch_tb(const TbArgs<T, C> &a0, hipStream_t st) {
+  TbArgs<T, C> a = a0;
   a.tiles_x = (a.nx + TXO - 1) / TXO;
   a.tiles_y = (a.ny + TYO - 1) / TYO;
   a.zchunk = (a.zchunk <= 0 || a.zchunk > a.nz) ? a.nz : a.zchunk;
   a.nzc = (a.nz + a.zchunk - 1) / a.zchunk;
-  hipLaunchKernelGGL((tb_kernel<T, KID, NT, TXO, TYO, TL>), dim3(a.tiles_x * a.tiles_y * a.nzc), dim3(NT), 0, st, a);
+  hipLaunchKernelGGL((tb_kernel<T, C, KID, NT, TXO, TYO, TL>), dim3(a.tiles_x * a.tiles_y * a.nzc), dim3(NT), 0, st, a);
   return hipGetLastError();
 }
 
@@ -275,26 +277,26 @@ int tb_depths(int kid, int dtype, int *out, int max) {
 
 // Tiles per kernel and precision, sized so the per-lane queues (levels x columns
 // x (2R+1) words) stay in registers without spilling.
-template <typename T>
-static hipError_t tb_dispatch(int kid, int tl, const TbArgs<T> &a, hipStream_t st) {
+template <typename T, typename C>
+static hipError_t tb_dispatch(int kid, int tl, const TbArgs<T, C> &a, hipStream_t st) {
   constexpr bool F32 = std::is_same<T, float>::value;
   constexpr int TYS = F32 ? 32 : 16;  // 64 x TYS output tile, 512 lanes
   if (kid == 5) {
-    if (tl == 2) return launch_tb<T, 5, 512, 64, TYS, 2>(a, st);
-    if (tl == 4) return launch_tb<T, 5, 512, 64, TYS, 4>(a, st);
-    if (tl == 6) return launch_tb<T, 5, 512, 64, 16, 6>(a, st);
+    if (tl == 2) return launch_tb<T, C, 5, 512, 64, TYS, 2>(a, st);
+    if (tl == 4) return launch_tb<T, C, 5, 512, 64, TYS, 4>(a, st);
+    if (tl == 6) return launch_tb<T, C, 5, 512, 64, 16, 6>(a, st);
   }
   return hipErrorInvalidValue;
 }
 
-template <typename T>
+template <typename T, typename C>
 static hipError_t tb_pass_t(const Problem &P, int tl, const void *Y, void *ZN, const void *src, long long src_idx,
                             const LaunchCfg &lc, hipStream_t st) {
   const long long sz = (long long)P.pitch * P.rows;
-  TbArgs<T> a{};
+  TbArgs<T, C> a{};
   a.Y = (const T *)Y;
   a.ZN = (T *)ZN;
-  a.coef = (const T *)P.coef;
+  a.coef = (const C *)P.coef;
   a.cstride = P.field_elems;
   a.sy = P.pitch;
   a.sz = sz;
@@ -306,7 +308,7 @@ static hipError_t tb_pass_t(const Problem &P, int tl, const void *Y, void *ZN, c
   a.src = (const T *)src;
   a.src_k = src_idx < 0 ? -1 : (int)(src_idx / sz);
   a.src_off = src_idx < 0 ? 0 : (int)(src_idx % sz);
-  return tb_dispatch<T>(P.kernel, tl, a, st);
+  return tb_dispatch<T, C>(P.kernel, tl, a, st);
 }
 
 // One fused pass of tl levels.  Kernels 1 and 5: Y -> ZN (X, ZO unused).
@@ -318,8 +320,9 @@ hipError_t tblock_pass(const Problem &P, int tl, const void *X, const void *Y, v
     return wave2_pass(P, X, Y, ZO, ZN, src, src_idx, lc, st);
   }
   if (P.kernel == 1) return jacobi_tb_pass(P, tl, Y, ZN, src, src_idx, lc, st);
-  if (P.dtype == GR_F64) return tb_pass_t<double>(P, tl, Y, ZN, src, src_idx, lc, st);
-  return tb_pass_t<float>(P, tl, Y, ZN, src, src_idx, lc, st);
+  if (P.dtype == GR_F64) return tb_pass_t<double, double>(P, tl, Y, ZN, src, src_idx, lc, st);
+  if (P.flags & GR_COEF_BF16) return tb_pass_t<float, __hip_bfloat16>(P, tl, Y, ZN, src, src_idx, lc, st);
+  return tb_pass_t<float, float>(P, tl, Y, ZN, src, src_idx, lc, st);
 }
 
 }  // namespace gr

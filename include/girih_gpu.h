@@ -151,8 +151,8 @@ int gr_init_coef_uniform(gr_handle *h, uint64_t seed, double lo0, double hi0, do
  * gr_get_stats for the resulting tile), z planes streamed per workgroup (0 = all),
  * fused time steps per HBM pass (1 = single sweep; kernel 0: 2; kernel 1: 2..6;
  * kernel 5: 2, 4, 6; other depths return GR_EUNSUPPORTED).  Fused passes run only
- * while U1 and U2 hold the same halo and the coefficients are not bf16; otherwise
- * the stepper uses single sweeps.  Values < 0 leave a setting unchanged.  Results
+ * while U1 and U2 hold the same halo, and with GR_COEF_BF16 for kernel 5 only;
+ * otherwise the stepper uses single sweeps.  Values < 0 leave a setting unchanged.  Results
  * never depend on the configuration. */
 int gr_set_config(gr_handle *h, int tile_variant, int zchunk, int t_block);
 /* Workgroup order of the sweep kernels: 1 = XCD-contiguous tile ranges, 0 = plain. */

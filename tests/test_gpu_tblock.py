@@ -14,7 +14,7 @@ import pytest
 import oracle as orc
 from girih_amd import gpu
 
-from test_gpu_parity import _consts, _pad, _pad_coef, _random_problem
+from test_gpu_parity import _bf16_round, _consts, _pad, _pad_coef, _random_problem
 
 pytestmark = pytest.mark.gpu
 
@@ -45,6 +45,53 @@ def test_fused_bitexact_vs_oracle(kernel, depth, prec):
             nnx = u.shape[2]
             np.testing.assert_array_equal(U1[:, :, :nnx], u)
             np.testing.assert_array_equal(U2[:, :, :nnx], v)
+
+
+@pytest.mark.parametrize("depth", [2, 4, 6])
+@pytest.mark.parametrize("zchunk", [0, 3])
+def test_fused_bf16_coefficients_bitexact_vs_oracle(depth, zchunk):
+    """Kernel 5 with its seven coefficient arrays stored as bf16 (GR_COEF_BF16):
+    fused passes widen each coefficient exactly to FP32 and keep girih's operand
+    order, so they are bit-identical to the FP32 oracle run on the bf16-rounded
+    coefficients, for whole-z and 3-plane chunks."""
+    kernel = 5
+    for n, (nx, ny, nz) in enumerate(SIZES[1][:2]):
+        napply = 2 * depth + 1
+        print("fused bf16", depth, zchunk, (nx, ny, nz), flush=True)
+        u, v, roc2, coef = _random_problem(kernel, nx, ny, nz, np.float32, 700 + n)
+        coef = _bf16_round(coef)
+        with gpu.Stepper(kernel, nx, ny, nz, dtype=gpu.GR_F32, flags=gpu.GR_COEF_BF16) as st:
+            st.upload(_pad(u), _pad(v), None, _pad_coef(coef, kernel))
+            st.set_config(zchunk=zchunk, t_block=depth)
+            st.step(napply)
+            assert st.stats()["last_launches"] < napply, "fused passes were not used"
+            U1, U2 = st.download(_pad(u).shape)
+        for a in range(napply):
+            if a % 2 == 0:
+                orc.sweep(kernel, coef, u, v, roc2)
+            else:
+                orc.sweep(kernel, coef, v, u, roc2)
+        nnx = u.shape[2]
+        np.testing.assert_array_equal(U1[:, :, :nnx], u)
+        np.testing.assert_array_equal(U2[:, :, :nnx], v)
+
+
+def test_bf16_roc2_wave_keeps_single_sweeps():
+    """The two-step wave pass reads roc2 in the field type only: with a bf16 roc2
+    field, kernel 0 stays on single sweeps (and on the oracle's results)."""
+    kernel, (nx, ny, nz) = 0, SIZES[4][0]
+    u, v, roc2, coef = _random_problem(kernel, nx, ny, nz, np.float32, 733)
+    roc2 = _bf16_round(roc2)
+    with gpu.Stepper(kernel, nx, ny, nz, dtype=gpu.GR_F32, flags=gpu.GR_COEF_BF16, coef=_consts(coef, kernel)) as st:
+        st.upload(_pad(u), _pad(v), _pad(roc2), None)
+        st.set_config(t_block=2)
+        st.step(4)
+        assert st.stats()["last_launches"] == 4
+        U1, U2 = st.download(_pad(u).shape)
+    orc.run(kernel, coef, u, v, roc2, nt=4)
+    nnx = u.shape[2]
+    np.testing.assert_array_equal(U1[:, :, :nnx], u)
+    np.testing.assert_array_equal(U2[:, :, :nnx], v)
 
 
 @pytest.mark.parametrize("kernel,depth", [(1, 2), (1, 3), (1, 5), (1, 6), (5, 4), (0, 2)])
