@@ -49,7 +49,9 @@ struct TbArgs {
   int src_k, src_off;
 };
 
-template <typename T, typename C, int KID, int NT, int TXO, int TYO, int TL>
+// CP: level t+1 takes plane p's coefficients from level t one step earlier (same lane
+// and column) instead of re-reading them; costs TL x 7 registers per column.
+template <typename T, typename C, int KID, int NT, int TXO, int TYO, int TL, bool CP>
 __global__ __launch_bounds__(NT) void tb_kernel(const TbArgs<T, C> a) {
   static_assert(KID == 5, "kernel 5 only");
   constexpr int R = kradius(KID);
@@ -152,11 +154,24 @@ __global__ __launch_bounds__(NT) void tb_kernel(const TbArgs<T, C> a) {
   };
   fetch0(kfirst);
 
+  // cs[t-1][j]: the coefficients level t uses in this step.  Level t+1 is live at
+  // (step k0+1, column j) only where level t was live at (k0, j) on the same plane p =
+  // k0 - t: level regions shrink with t and level t+1 starts R planes later.
+  T cs[CP ? TL : 1][NCOL][7];
+
   for (int kb = kfirst; kb < klast; kb += Q) {
 #pragma unroll
     for (int ph = 0; ph < Q; ++ph) {
       const int k0 = kb + ph;
       if (k0 >= klast) break;
+      if constexpr (CP) {
+#pragma unroll
+        for (int t = TL - 1; t >= 1; --t)
+#pragma unroll
+          for (int j = 0; j < NCOL; ++j)
+#pragma unroll
+            for (int m = 0; m < 7; ++m) cs[t][j][m] = cs[t - 1][j][m];
+      }
       // slot of plane p: ((p mod Q) + Q) mod Q; k0 - kfirst = kb - kfirst + ph and
       // (kb - kfirst) is a multiple of Q, so plane k0 - R*t sits in slot
       // (ph - R*t) mod Q relative to kfirst's slot 0.
@@ -207,9 +222,19 @@ __global__ __launch_bounds__(NT) void tb_kernel(const TbArgs<T, C> a) {
               const T v0 = zq[R];
               {  // stencils.c:193-201; coefficients re-read by every level stay cached
                 const C *cp = a.coef + (long long)p * a.sz + off[j] / sizeof(T);
-                auto cf = [&](int m) { return ld_coef<T, C, false>(cp + m * a.cstride); };
-                out = cf(0) * v0 + cf(1) * xn(-1) + cf(2) * xn(1) + cf(3) * yn(-1) + cf(4) * yn(1) +
-                      cf(5) * zq[0] + cf(6) * zq[2];
+                if constexpr (CP) {
+                  if (t == 1) {
+#pragma unroll
+                    for (int m = 0; m < 7; ++m) cs[0][j][m] = ld_coef<T, C, false>(cp + m * a.cstride);
+                  }
+                  const T *c = cs[CP ? t - 1 : 0][j];
+                  out = c[0] * v0 + c[1] * xn(-1) + c[2] * xn(1) + c[3] * yn(-1) + c[4] * yn(1) + c[5] * zq[0] +
+                        c[6] * zq[2];
+                } else {
+                  auto cf = [&](int m) { return ld_coef<T, C, false>(cp + m * a.cstride); };
+                  out = cf(0) * v0 + cf(1) * xn(-1) + cf(2) * xn(1) + cf(3) * yn(-1) + cf(4) * yn(1) +
+                        cf(5) * zq[0] + cf(6) * zq[2];
+                }
               }
               if (p == a.src_k && off[j] == (unsigned)a.src_off * (unsigned)sizeof(T)) out = out + a.src[t - 1];
             } else {
@@ -245,14 +270,14 @@ __global__ __launch_bounds__(NT) void tb_kernel(const TbArgs<T, C> a) {
   }
 }
 
-template <typename T, typename C, int KID, int NT, int TXO, int TYO, int TL>
+template <typename T, typename C, int KID, int NT, int TXO, int TYO, int TL, bool CP = false>
 static hipError_t launch_tb(const TbArgs<T, C> &a0, hipStream_t st) {
   TbArgs<T, C> a = a0;
   a.tiles_x = (a.nx + TXO - 1) / TXO;
   a.tiles_y = (a.ny + TYO - 1) / TYO;
   a.zchunk = (a.zchunk <= 0 || a.zchunk > a.nz) ? a.nz : a.zchunk;
   a.nzc = (a.nz + a.zchunk - 1) / a.zchunk;
-  hipLaunchKernelGGL((tb_kernel<T, C, KID, NT, TXO, TYO, TL>), dim3(a.tiles_x * a.tiles_y * a.nzc), dim3(NT), 0, st, a);
+  hipLaunchKernelGGL((tb_kernel<T, C, KID, NT, TXO, TYO, TL, CP>), dim3(a.tiles_x * a.tiles_y * a.nzc), dim3(NT), 0, st, a);
   return hipGetLastError();
 }
 
@@ -281,8 +306,11 @@ template <typename T, typename C>
 static hipError_t tb_dispatch(int kid, int tl, const TbArgs<T, C> &a, hipStream_t st) {
   constexpr bool F32 = std::is_same<T, float>::value;
   constexpr int TYS = F32 ? 32 : 16;  // 64 x TYS output tile, 512 lanes
+  // depth 2 hands the coefficients from level 1 to level 2 (2 x 7 values per column
+  // held in registers, no spills); at depths 4 and 6 the hand-off would spill, so
+  // those levels re-read them
   if (kid == 5) {
-    if (tl == 2) return launch_tb<T, C, 5, 512, 64, TYS, 2>(a, st);
+    if (tl == 2) return launch_tb<T, C, 5, 512, 64, TYS, 2, true>(a, st);
     if (tl == 4) return launch_tb<T, C, 5, 512, 64, TYS, 4>(a, st);
     if (tl == 6) return launch_tb<T, C, 5, 512, 64, 16, 6>(a, st);
   }
