@@ -305,12 +305,17 @@ int tb_depths(int kid, int dtype, int *out, int max) {
 template <typename T, typename C>
 static hipError_t tb_dispatch(int kid, int tl, const TbArgs<T, C> &a, hipStream_t st) {
   constexpr bool F32 = std::is_same<T, float>::value;
-  constexpr int TYS = F32 ? 32 : 16;  // 64 x TYS output tile, 512 lanes
+  constexpr int TYS = F32 ? 32 : 16;  // 64 x TYS output tile
   // depth 2 hands the coefficients from level 1 to level 2 (2 x 7 values per column
   // held in registers, no spills); at depths 4 and 6 the hand-off would spill, so
   // those levels re-read them
   if (kid == 5) {
-    if (tl == 2) return launch_tb<T, C, 5, 512, 64, TYS, 2, true>(a, st);
+    // FP64: 640 lanes give 2 columns per lane (66 x 18 level-1 cells), 112 VGPRs and 4
+    // waves per SIMD, against 3 columns, 156 VGPRs and 3 waves with 512 lanes
+    if (tl == 2) {
+      if constexpr (F32) return launch_tb<T, C, 5, 512, 64, TYS, 2, true>(a, st);
+      else return launch_tb<T, C, 5, 640, 64, 16, 2, true>(a, st);
+    }
     if (tl == 4) return launch_tb<T, C, 5, 512, 64, TYS, 4>(a, st);
     if (tl == 6) return launch_tb<T, C, 5, 512, 64, 16, 6>(a, st);
   }
