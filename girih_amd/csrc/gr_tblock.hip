@@ -316,7 +316,13 @@ static hipError_t tb_dispatch(int kid, int tl, const TbArgs<T, C> &a, hipStream_
       if constexpr (F32) return launch_tb<T, C, 5, 512, 64, TYS, 2, true>(a, st);
       else return launch_tb<T, C, 5, 640, 64, 16, 2, true>(a, st);
     }
-    if (tl == 4) return launch_tb<T, C, 5, 512, 64, TYS, 4>(a, st);
+    // FP32 coefficients at depth 4: 896 lanes, 3 columns per lane (70 x 38 level-1 cells),
+    // 70 VGPRs (512 lanes: 6 columns, 119 VGPRs); with bf16 coefficients 512 lanes measured
+    // faster, and at depth 6 1024 lanes were slower
+    if (tl == 4) {
+      if constexpr (std::is_same<C, float>::value) return launch_tb<T, C, 5, 896, 64, 32, 4>(a, st);
+      else return launch_tb<T, C, 5, 512, 64, TYS, 4>(a, st);
+    }
     if (tl == 6) return launch_tb<T, C, 5, 512, 64, 16, 6>(a, st);
   }
   return hipErrorInvalidValue;
