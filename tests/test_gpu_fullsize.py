@@ -9,8 +9,8 @@
   sums), which the oracle cannot check at this size in reasonable time;
 * 1^3 .. 3^3 grids for every kernel: bit-identical to the oracle;
 * fused passes at the benchmark sizes (7-point Jacobi 512^3 FP32 depth 4 and FP64
-  depth 3, 25-point wave 1024^3 FP32 two-step passes): bit-identical to single
-  sweeps, compared on the device.
+  depth 3, 25-point wave 1024^3 FP32 two-step passes, 7-point var-coef 512^3 FP32
+  depth 4 and FP64 depth 2): bit-identical to single sweeps, compared on the device.
 """
 import numpy as np
 import pytest
@@ -98,18 +98,23 @@ def test_smallest_grids_bitexact(kernel):
             np.testing.assert_array_equal(U2[:, :, :u.shape[2]], v, err_msg="%s %d" % (prec, n))
 
 
-@pytest.mark.parametrize("kernel,n,prec,depth", [(1, 512, "sp", 4), (1, 512, "dp", 3), (0, 1024, "sp", 2)])
+@pytest.mark.parametrize("kernel,n,prec,depth", [(1, 512, "sp", 4), (1, 512, "dp", 3), (0, 1024, "sp", 2),
+                                                  (5, 512, "sp", 4), (5, 512, "dp", 2)])
 def test_fused_equals_sweeps_at_benchmark_size(kernel, n, prec, depth):
     dtype = gpu.GR_F32 if prec == "sp" else gpu.GR_F64
     if kernel == 0:
         kw = dict(flags=gpu.GR_FAST | gpu.GR_ROC2_SCALAR, roc2=0.16, coef=physics.fd8_coef())
-    else:
+    elif kernel == 1:
         kw = dict(coef=[0.4, 0.1])
+    else:
+        kw = {}
     napply = 2 * depth + (0 if kernel == 0 else 1)  # first-order kernels end on a single sweep
     print("fused at size", kernel, n, prec, depth, flush=True)
     with gpu.Stepper(kernel, n, n, n, dtype=dtype, **kw) as a, gpu.Stepper(kernel, n, n, n, dtype=dtype, **kw) as b:
         for st in (a, b):
             st.init_fields(1, 4321)  # interior U(0,1), U1 = U2, zero halos
+            if kernel == 5:
+                st.init_coef_uniform(5, 0.3, 0.5, 0.0, 0.1)
         a.set_config(13, 64, 1)
         b.set_config(13, 256, depth)
         a.step(napply)
