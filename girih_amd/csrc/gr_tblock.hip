@@ -310,10 +310,12 @@ static hipError_t tb_dispatch(int kid, int tl, const TbArgs<T, C> &a, hipStream_
   // held in registers, no spills); at depths 4 and 6 the hand-off would spill, so
   // those levels re-read them
   if (kid == 5) {
+    // FP32: 896 lanes give 3 columns per lane (66 x 34 level-1 cells), 86 VGPRs and 5
+    // waves per SIMD, against 5 columns, 139 VGPRs and 3 waves with 512 lanes.
     // FP64: 640 lanes give 2 columns per lane (66 x 18 level-1 cells), 112 VGPRs and 4
     // waves per SIMD, against 3 columns, 156 VGPRs and 3 waves with 512 lanes
     if (tl == 2) {
-      if constexpr (F32) return launch_tb<T, C, 5, 512, 64, TYS, 2, true>(a, st);
+      if constexpr (F32) return launch_tb<T, C, 5, 896, 64, 32, 2, true>(a, st);
       else return launch_tb<T, C, 5, 640, 64, 16, 2, true>(a, st);
     }
     // FP32 coefficients at depth 4: 896 lanes, 3 columns per lane (70 x 38 level-1 cells),
