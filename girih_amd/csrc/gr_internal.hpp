@@ -15,6 +15,7 @@ struct Problem {
   long long field_elems = 0;  // elements of one field allocation
   double c[5] = {0, 0, 0, 0, 0};
   double roc2s = 0;
+  int strm_mode = -1;    // non-temporal sweep instantiations: -1 by working-set size, 0 never, 1 always
   void *roc2 = nullptr;  // device roc2 field base (x offset applied)
   void *coef = nullptr;  // device coefficient arrays base (x offset applied)
 };
@@ -30,8 +31,9 @@ int num_tile_variants(int kid, int dtype);
 void tile_of_variant(int kid, int dtype, int variant, int *tx, int *ty);
 
 size_t elem_size(int dtype);
-// Whether a sweep of P should use non-temporal hints: its fields and coefficient
-// arrays (the data one step streams) exceed what the caches keep between steps.
+// Whether a sweep of P should use non-temporal hints: by default, when its fields and
+// coefficient arrays (the data one step streams) exceed what the caches keep between
+// steps; P.strm_mode 0 / 1 forces the choice.
 int streaming_sweep(const Problem &P);
 int set_error(int code, const char *msg);
 int set_errorf(int code, const char *fmt, ...) __attribute__((format(printf, 2, 3)));

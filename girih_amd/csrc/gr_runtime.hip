@@ -61,6 +61,7 @@ int set_errorf(int code, const char *fmt, ...) {
 size_t elem_size(int dtype) { return dtype == GR_F64 ? 8 : 4; }
 
 int streaming_sweep(const Problem &P) {
+  if (P.strm_mode >= 0) return P.strm_mode;
   const double cbytes = (P.flags & GR_COEF_BF16) ? 2.0 : (double)elem_size(P.dtype);
   const double words = 2.0 * elem_size(P.dtype) + cbytes * (P.ncoef + (P.roc2 ? 1 : 0));
   return (double)P.field_elems * words > 512.0 * 1024 * 1024 ? 1 : 0;  // 2x the 256 MB Infinity Cache
@@ -266,7 +267,17 @@ extern "C" int gr_create(const gr_desc *d, gr_handle **out) {
     const char *e = getenv("GIRIH_XCD_ORDER");
     h->lc.xcd_order = (e && e[0] == '1') ? 1 : 0;
   }
-  h->lc.t_block = d->t_block <= 0 ? 1 : d->t_block;
+  h->lc.t_block = d->t_block <= 0 ? 1 : d->t_block;  // 0: single sweeps until gr_autotune picks a depth
+  if (h->lc.t_block > 1) {
+    int depths[8];
+    const int nd = tb_depths(P.kernel, P.dtype, depths, 8);
+    bool have = false;
+    for (int i = 0; i < nd; ++i) have |= depths[i] == h->lc.t_block;
+    if (!have) {
+      delete h;
+      return fail(GR_EUNSUPPORTED, "gr_create: fused depth %d not available for kernel %d", d->t_block, d->kernel);
+    }
+  }
   const size_t fbytes = (size_t)P.field_elems * es;
   hipError_t e;
   int rc = GR_OK;
@@ -356,6 +367,11 @@ extern "C" int gr_upload(gr_handle *h, const int ld[3], const void *U1, const vo
   if (rc) return rc;
   const Problem &P = h->P;
   const size_t es = elem_size(P.dtype);
+  if (U1 && U2) {  // both fields are overwritten: back to the first buffer pair
+    drop_graph(h);
+    h->lu[0] = 0;
+    h->lu[1] = 1;
+  }
   if (U1 && (rc = copy_field(h, ld, ubase(h, 0), es, U1, es, true))) return rc;
   if (U2 && (rc = copy_field(h, ld, ubase(h, 1), es, U2, es, true))) return rc;
   const size_t hstride = (size_t)ld[0] * ld[1] * ld[2] * es;
@@ -675,6 +691,14 @@ extern "C" int gr_set_config(gr_handle *h, int variant, int zchunk, int t_block)
   return GR_OK;
 }
 
+extern "C" int gr_set_streaming(gr_handle *h, int mode) {
+  if (!h) return fail(GR_EINVAL, "gr_set_streaming: null handle");
+  if (mode < -1 || mode > 1) return fail(GR_EINVAL, "gr_set_streaming: mode must be -1, 0 or 1");
+  if (mode != h->P.strm_mode) drop_graph(h);  // a captured sequence holds the other instantiations
+  h->P.strm_mode = mode;
+  return GR_OK;
+}
+
 extern "C" int gr_set_xcd_order(gr_handle *h, int xcd_order) {
   if (!h) return fail(GR_EINVAL, "gr_set_xcd_order: null handle");
   if (xcd_order != 0 && xcd_order != 1) return fail(GR_EINVAL, "gr_set_xcd_order: 0 or 1");
@@ -871,7 +895,7 @@ hipError_t sweep_box(const Problem &P, const Geom &g, void *u, const void *v, co
     for (int r = 0; r < 5; ++r) a.c[r] = P.c[r];
     a.g = g;
     a.src_k = -1;
-    return launch_sweep_f64(P.kernel, false, false, a, lc, st, false);
+    return launch_sweep_f64(P.kernel, false, false, a, lc, st, streaming_sweep(P));
   }
   SweepArgs<float, float> a{};
   a.u = (float *)u;
@@ -882,7 +906,7 @@ hipError_t sweep_box(const Problem &P, const Geom &g, void *u, const void *v, co
   for (int r = 0; r < 5; ++r) a.c[r] = (float)P.c[r];
   a.g = g;
   a.src_k = -1;
-  return launch_sweep_f32(P.kernel, false, false, a, lc, st, false);
+  return launch_sweep_f32(P.kernel, false, false, a, lc, st, streaming_sweep(P));
 }
 }  // namespace gr
 
@@ -995,6 +1019,11 @@ extern "C" int gr_init_fields(gr_handle *h, int mode, uint64_t seed) {
   const long long total = (long long)(P.nx + 2 * P.R) * P.rows * P.planes;
   const int bs = 256;
   const unsigned nb = (unsigned)((total + bs - 1) / bs);
+  if (h->lu[0] != 0 || h->lu[1] != 1) {  // both fields are rewritten: back to the first buffer pair
+    drop_graph(h);
+    h->lu[0] = 0;
+    h->lu[1] = 1;
+  }
   void *b1 = ubase(h, 0), *b2 = ubase(h, 1);
   if (P.dtype == GR_F64) {
     hipLaunchKernelGGL(init_fields_kernel<double>, dim3(nb), dim3(bs), 0, h->stream, (double *)b1, (double *)b2,

@@ -422,7 +422,7 @@ __device__ __forceinline__ void SweepTile<T, C, KID, BY, RY, FAST, ROCS, RING, N
       __syncthreads();
 
       T *__restrict__ uk = a.u + (long long)k * g.sz;
-      const bool src_plane = (k == a.src_k);
+      const bool src_plane = a.src_k >= 0 && k == a.src_k;  // src_k = -1: no source
 #pragma unroll
       for (int m = 0; m < RY; ++m) {
         const int ly = R + ty * RY + m, lx = R + tx;

@@ -236,7 +236,7 @@ __global__ __launch_bounds__(NT) void tb_kernel(const TbArgs<T, C> a) {
                         cf(5) * zq[0] + cf(6) * zq[2];
                 }
               }
-              if (p == a.src_k && off[j] == (unsigned)a.src_off * (unsigned)sizeof(T)) out = out + a.src[t - 1];
+              if (a.src_k >= 0 && p == a.src_k && off[j] == (unsigned)a.src_off * (unsigned)sizeof(T)) out = out + a.src[t - 1];
             } else {
               out = ldb<T>(yp, off[j]);  // girih's halo: never updated
             }

@@ -67,7 +67,8 @@ class BatchStats(ctypes.Structure):
 
 EXPORTS = ["gr_create", "gr_destroy", "gr_last_error", "gr_upload", "gr_download", "gr_step", "gr_run_ts",
            "gr_set_source", "gr_get_stats", "gr_mwd_round_nt", "gr_sweep_dev", "gr_field_ptr", "gr_init_fields",
-           "gr_init_coef_uniform", "gr_compare_fields", "gr_set_config", "gr_set_xcd_order", "gr_set_graph", "gr_autotune",
+           "gr_init_coef_uniform", "gr_compare_fields", "gr_set_config", "gr_set_xcd_order", "gr_set_streaming",
+           "gr_set_graph", "gr_autotune",
            "gr_init_roc2_model",
            "gr_download_roc2", "gr_batch_create", "gr_batch_destroy", "gr_batch_problem", "gr_batch_set_config",
            "gr_batch_step", "gr_batch_autotune", "gr_batch_get_stats", "gr_shot_create", "gr_shot_destroy",
@@ -110,6 +111,7 @@ def lib():
             "gr_compare_fields": ([P, I, P, I, ctypes.POINTER(ctypes.c_double)], I),
             "gr_set_config": ([P, I, I, I], I),
             "gr_set_xcd_order": ([P, I], I),
+            "gr_set_streaming": ([P, I], I),
             "gr_set_graph": ([P, I], I),
             "gr_autotune": ([P, I, ctypes.POINTER(Stats)], I),
             "gr_init_roc2_model": ([P, ctypes.POINTER(VModel)], I),
@@ -281,6 +283,11 @@ class Stepper:
 
     def set_xcd_order(self, order):
         check(lib().gr_set_xcd_order(self._h, int(order)))
+
+    def set_streaming(self, mode):
+        """Cache policy: -1 by working-set size (default), 0 normal accesses, 1 non-temporal
+        loads and stores for the data a step touches once (results do not change)."""
+        check(lib().gr_set_streaming(self._h, int(mode)))
 
     def set_graph(self, on=True):
         """Replay gr_step's launch sequence from a HIP graph (small, launch-bound grids)."""

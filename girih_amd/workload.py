@@ -56,6 +56,10 @@ def main():
     ap.add_argument("--batch", action="store_true", help="the 16-problem bf16 wave batch")
     ap.add_argument("--kernel", type=int)
     ap.add_argument("--n", type=int)
+    ap.add_argument("--shape", default=None, help="NX,NY,NZ (instead of --n for all three)")
+    ap.add_argument("--strm", type=int, choices=[-1, 0, 1], default=-1,
+                    help="cache policy (gr_set_streaming): -1 by working-set size, 0 normal, 1 non-temporal")
+    ap.add_argument("--graph", action="store_true", help="replay the timed steps from a HIP graph")
     ap.add_argument("--dtype", choices=["f32", "f64"], default="f32")
     ap.add_argument("--bf16", action="store_true", help="coefficient arrays / roc2 stored as bf16")
     ap.add_argument("--config", default=None)
@@ -66,8 +70,9 @@ def main():
     a = ap.parse_args()
     if a.batch:
         return batch_main(a)
-    if a.kernel is None or a.n is None:
-        ap.error("--kernel and --n are required (or --batch)")
+    if a.kernel is None or (a.n is None and a.shape is None):
+        ap.error("--kernel and --n (or --shape) are required (or --batch)")
+    nx, ny, nz = (int(x) for x in a.shape.split(",")) if a.shape else (a.n, a.n, a.n)
     dtype = gpu.GR_F32 if a.dtype == "f32" else gpu.GR_F64
     flags = gpu.GR_COEF_BF16 if a.bf16 else 0
     coef = None
@@ -76,7 +81,8 @@ def main():
         coef = physics.fd8_coef()
     elif a.kernel == 1:
         coef = [0.4, 0.1]
-    st = gpu.Stepper(a.kernel, a.n, a.n, a.n, dtype=dtype, flags=flags, roc2=0.16, coef=coef)
+    st = gpu.Stepper(a.kernel, nx, ny, nz, dtype=dtype, flags=flags, roc2=0.16, coef=coef)
+    st.set_streaming(a.strm)
     if gpu.NUM_COEF_ARRAYS[a.kernel]:
         st.init_coef_uniform(5, 0.3, 0.5, 0.0, 0.1)
     st.init_fields({"girih": 0, "random": 1, "zero": 2}[a.init], 12345)
@@ -92,12 +98,16 @@ def main():
     napply = a.steps * tb + (0 if a.kernel == 0 or tb == 1 else 1)
     if a.warmup:
         st.step(tb if a.kernel == 0 or tb == 1 else tb + 1)  # loads the code objects
+    if a.graph:
+        napply += napply % 2  # an even count restores the parity, so the timed call replays the capture
+        st.set_graph(True)
+        st.step(napply)  # captures the sequence
     st.step(napply)
     s = st.stats()
-    print(json.dumps({"kernel": a.kernel, "n": a.n, "dtype": a.dtype, "bf16": a.bf16, "init": a.init,
+    print(json.dumps({"kernel": a.kernel, "shape": [nx, ny, nz], "dtype": a.dtype, "bf16": a.bf16, "init": a.init,
                       "config": "%d,%d,%d,%d" % (s["tile_variant"], s["zchunk"], s["xcd_order"], s["t_block"]),
-                      "applications": napply, "launches": s["last_launches"],
-                      "glups": a.n ** 3 * napply / s["last_ms"] / 1e6}))
+                      "strm": a.strm, "graph": a.graph, "applications": napply, "launches": s["last_launches"],
+                      "ms": s["last_ms"], "glups": nx * ny * nz * napply / s["last_ms"] / 1e6}))
     st.close()
 
 

@@ -58,7 +58,8 @@ typedef struct gr_desc {
   int dtype;        /* GR_F64 / GR_F32 */
   int nx, ny, nz;   /* interior size (girih --nx --ny --nz) */
   int flags;        /* GR_ROC2_SCALAR | GR_FAST | GR_COEF_BF16 */
-  int t_block;      /* time steps fused per HBM pass: 0 = autotuned, 1 = single sweep */
+  int t_block;      /* time steps fused per HBM pass: 0 or 1 = single sweeps (gr_autotune may
+                       then pick a depth); a depth gr_set_config rejects -> GR_EUNSUPPORTED */
   int reserved;
   double roc2;      /* constant roc2 when GR_ROC2_SCALAR */
   double coef[11];  /* constant coefficients, kernels 0 and 1 (girih coef[0..R]) */
@@ -157,6 +158,12 @@ int gr_init_coef_uniform(gr_handle *h, uint64_t seed, double lo0, double hi0, do
 int gr_set_config(gr_handle *h, int tile_variant, int zchunk, int t_block);
 /* Workgroup order of the sweep kernels: 1 = XCD-contiguous tile ranges, 0 = plain. */
 int gr_set_xcd_order(gr_handle *h, int xcd_order);
+/* Cache policy of the sweeps and fused passes (no counterpart in girih): mode -1
+ * (default) uses non-temporal loads and stores for the data a step touches once when
+ * the problem's working set exceeds 512 MB (twice the Infinity Cache), and normal
+ * accesses otherwise; 0 never uses them; 1 always does.  The arithmetic is the same
+ * in every mode: results never depend on it. */
+int gr_set_streaming(gr_handle *h, int mode);
 /* Graph mode (no counterpart in girih): gr_step captures its launch sequence into a
  * HIP graph once and replays it while the step count, parity, configuration and
  * source window repeat (one launch per call instead of one per sweep; for small,

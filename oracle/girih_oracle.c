@@ -64,3 +64,61 @@ int gro_mwd_round_nt(int nt, int t_dim) {
 #undef SFX
 #undef TWO
 #undef FABS
+
+/*
+ * FP32 kernel 0 in the library's GR_FAST arithmetic (no girih counterpart; girih's
+ * FP32 build uses the operand order of k0_wave25 above).  Radii 1-2 are summed as
+ * differences from the centre, radii 3-4 as pair sums, each radius scaled with one
+ * correctly rounded fused multiply-add, from radius 4 down to 1; the centre weight
+ * c0 + 6*(c1 + c2) is rounded once from double; the update is the increment form
+ *   u = v + fma(roc2, lap, v - u).
+ * The oracle keeps this as its own function so that tests can compare the GPU's
+ * GR_FAST kernels bit for bit; the function itself is checked against the FP64
+ * reference order (k0_wave25_f64) within a stated tolerance and against the exact
+ * Laplacian of a quadratic field.  roc2 == NULL uses the scalar roc2s.
+ */
+void gro_sweep_fast_f32(const int shape[3], const double coef[5], float *u, const float *v,
+                        const float *roc2, double roc2s) {
+  const int nnx = shape[0], nny = shape[1], nnz = shape[2];
+  const uint64_t sy = (uint64_t)nnx, sz = (uint64_t)nnx * nny;
+  float c[5];
+  double s = coef[0];
+  int r, i, j, k;
+  for (r = 0; r < 5; r++) c[r] = (float)coef[r];
+  s += 6.0 * coef[1];
+  s += 6.0 * coef[2];
+  const float csum = (float)s, rs = (float)roc2s;
+  for (k = 4; k < nnz - 4; k++)
+    for (j = 4; j < nny - 4; j++)
+      for (i = 4; i < nnx - 4; i++) {
+        const uint64_t o = (uint64_t)k * sz + (uint64_t)j * sy + (uint64_t)i;
+        const float v0 = v[o];
+        float l = csum * v0;
+        for (r = 4; r >= 1; r--) {
+          const float xp = v[o + r], xm = v[o - r];
+          const float yp = v[o + r * sy], ym = v[o - r * sy];
+          const float zp = v[o + r * sz], zm = v[o - r * sz];
+          float d;
+          if (r <= 2)
+            d = (((xp - v0) + (xm - v0)) + ((yp - v0) + (ym - v0))) + ((zp - v0) + (zm - v0));
+          else
+            d = ((xp + xm) + (yp + ym)) + (zp + zm);
+          l = fmaf(c[r], d, l);
+        }
+        const float rc = roc2 ? roc2[o] : rs;
+        u[o] = v0 + fmaf(rc, l, v0 - u[o]);
+      }
+}
+
+/* The time loop of gro_run_f32 on gro_sweep_fast_f32 (same source injection). */
+int gro_run_fast_f32(const int shape[3], const double coef[5], float *u, float *v, const float *roc2,
+                     double roc2s, int nt, int64_t src_idx, const float *src) {
+  int it;
+  for (it = 0; it < nt; it += 2) {
+    gro_sweep_fast_f32(shape, coef, u, v, roc2, roc2s);
+    if (src_idx >= 0 && src) u[src_idx] += src[it];
+    gro_sweep_fast_f32(shape, coef, v, u, roc2, roc2s);
+    if (src_idx >= 0 && src) v[src_idx] += src[it + 1];
+  }
+  return 2 * ((nt + 1) / 2);
+}

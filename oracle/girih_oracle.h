@@ -73,6 +73,18 @@ int gro_compare_f32(const float *u, const float *target, int nx, int ny, int nz,
                     double *emax, double *el1);
 
 /*
+ * FP32 kernel 0 in the library's GR_FAST arithmetic (no girih counterpart): per-radius
+ * sums (radii 1-2 as differences from the centre) scaled with fmaf from radius 4 to 1,
+ * centre weight (float)(c0 + 6*(c1 + c2)) rounded once from double, increment-form
+ * update u = v + fmaf(roc2, lap, v - u).  roc2 == NULL uses the scalar roc2s.
+ * gro_run_fast_f32 is gro_run_f32's time loop (with its source) on this sweep.
+ */
+void gro_sweep_fast_f32(const int shape[3], const double coef[5], float *u, const float *v,
+                        const float *roc2, double roc2s);
+int gro_run_fast_f32(const int shape[3], const double coef[5], float *u, float *v, const float *roc2,
+                     double roc2s, int nt, int64_t src_idx, const float *src);
+
+/*
  * Seismic shot on kernel 0 in FP64 (adjoint.c; no girih counterpart): forward
  * traces of plane rec_k (trace [nt][ny][nx], may be NULL) and, when res and grad
  * are given, the adjoint gradient of sum_a <res[a], trace[a]> with respect to the

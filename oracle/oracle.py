@@ -51,6 +51,10 @@ def lib():
             f = getattr(L, "gro_default_coef_" + sfx)
             f.argtypes = [P]
             f.restype = None
+        L.gro_sweep_fast_f32.argtypes = [P, P, P, P, P, ctypes.c_double]
+        L.gro_sweep_fast_f32.restype = None
+        L.gro_run_fast_f32.argtypes = [P, P, P, P, P, ctypes.c_double, I, ctypes.c_int64, P]
+        L.gro_run_fast_f32.restype = I
         L.gro_mwd_round_nt.argtypes = [I, I]
         L.gro_mwd_round_nt.restype = I
         L.gro_shot_f64.argtypes = [P, P, P, I, ctypes.c_int64, P, I, P, P, P, ctypes.c_double]
@@ -136,6 +140,33 @@ def run(kernel, coef, u, v, roc2=None, nt=2, src_idx=None, src=None):
     if n < 0:
         raise ValueError("bad kernel id %d" % kernel)
     return n
+
+
+def sweep_fast(coef, u, v, roc2=None, roc2s=0.0):
+    """FP32 kernel 0 in the library's GR_FAST arithmetic (u <- f(v) in place on u);
+    coef holds c0..c4 (taken as float64 and rounded like the library does); roc2 is a
+    float32 array or None for the scalar roc2s."""
+    assert u.dtype == np.float32 and v.dtype == np.float32
+    c = np.ascontiguousarray(np.asarray(coef, dtype=np.float64)[:5])
+    shape = (ctypes.c_int * 3)(u.shape[2], u.shape[1], u.shape[0])
+    r = None if roc2 is None else np.ascontiguousarray(roc2, dtype=np.float32)
+    lib().gro_sweep_fast_f32(shape, _ptr(c), _ptr(u), _ptr(v), _ptr(r), float(roc2s))
+
+
+def run_fast(coef, u, v, roc2=None, roc2s=0.0, nt=2, src_idx=None, src=None):
+    """run() for the GR_FAST sweep (FP32 kernel 0); returns applied steps."""
+    assert u.dtype == np.float32 and v.dtype == np.float32
+    c = np.ascontiguousarray(np.asarray(coef, dtype=np.float64)[:5])
+    shape = (ctypes.c_int * 3)(u.shape[2], u.shape[1], u.shape[0])
+    r = None if roc2 is None else np.ascontiguousarray(roc2, dtype=np.float32)
+    flat = -1
+    if src_idx is not None:
+        i, j, k = src_idx
+        flat = (k * u.shape[1] + j) * u.shape[2] + i
+        src = np.ascontiguousarray(src, dtype=np.float32)
+        assert src.size >= 2 * ((nt + 1) // 2)
+    return lib().gro_run_fast_f32(shape, _ptr(c), _ptr(u), _ptr(v), _ptr(r), float(roc2s), int(nt), flat,
+                                  _ptr(src) if flat >= 0 else None)
 
 
 def compare(u, target, R):

@@ -99,15 +99,18 @@ def test_smallest_grids_bitexact(kernel):
 
 
 @pytest.mark.parametrize("kernel,n,prec,depth", [(1, 512, "sp", 4), (1, 512, "dp", 3), (0, 1024, "sp", 2),
-                                                  (5, 512, "sp", 4), (5, 512, "dp", 2)])
+                                                  (5, 512, "sp", 4), (5, 512, "dp", 2), (5, 512, "sp", 2),
+                                                  (5, 512, "bf16", 2), (5, 512, "bf16", 4)])
 def test_fused_equals_sweeps_at_benchmark_size(kernel, n, prec, depth):
-    dtype = gpu.GR_F32 if prec == "sp" else gpu.GR_F64
+    """Includes the passes the autotuner picks at these sizes: kernel 5 at depth 2 with
+    the coefficient hand-off (FP32 on 896 lanes, FP64 on 640 lanes, bf16 coefficients)."""
+    dtype = gpu.GR_F64 if prec == "dp" else gpu.GR_F32
     if kernel == 0:
         kw = dict(flags=gpu.GR_FAST | gpu.GR_ROC2_SCALAR, roc2=0.16, coef=physics.fd8_coef())
     elif kernel == 1:
         kw = dict(coef=[0.4, 0.1])
     else:
-        kw = {}
+        kw = dict(flags=gpu.GR_COEF_BF16) if prec == "bf16" else {}
     napply = 2 * depth + (0 if kernel == 0 else 1)  # first-order kernels end on a single sweep
     print("fused at size", kernel, n, prec, depth, flush=True)
     with gpu.Stepper(kernel, n, n, n, dtype=dtype, **kw) as a, gpu.Stepper(kernel, n, n, n, dtype=dtype, **kw) as b:
@@ -124,3 +127,41 @@ def test_fused_equals_sweeps_at_benchmark_size(kernel, n, prec, depth):
             c = b.compare(f, a, f)
             assert c["max_ref"] > 0
             assert c["max_diff"] == 0.0 and c["sq_diff"] == 0.0, (f, c)
+
+
+def test_wave_1024_fp32_meets_error_and_speed_targets_after_1000_steps():
+    """The 25-point wave at 1024^3 (v = 3000 m/s, h = 10 m, CFL 0.4, 15 Hz Ricker at the
+    centre, zero initial fields), 1000 steps, each precision on its autotuned
+    configuration: the FP32 GR_FAST path stays within 1e-5 relative L2 of the FP64
+    path (girih's operand order, bit-identical to the oracle) and runs at least 1.8x
+    its lattice updates per second."""
+    import time
+
+    import torch
+
+    n, nt = 1024, 1000
+    cfg = physics.wave_1024_config(nx=n, ny=n, nz=n, nt=nt)
+    runs = {}
+    for name, dtype, flags in (("f32", gpu.GR_F32, gpu.GR_FAST), ("f64", gpu.GR_F64, 0)):
+        print("wave 1024 x 1000", name, flush=True)
+        st = gpu.Stepper(0, n, n, n, dtype=dtype, flags=flags | gpu.GR_ROC2_SCALAR, roc2=cfg["roc2"], coef=cfg["coef"])
+        st.autotune(4)
+        st.init_fields(2)
+        st.step(4)  # untimed: loads the tuned kernels' code
+        st.init_fields(2)
+        st.set_source(*cfg["src_point"], cfg["src"][:nt + 2])
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        st.step(nt)
+        torch.cuda.synchronize()
+        runs[name] = (st, n ** 3 * nt / (time.perf_counter() - t0) / 1e9)
+    f = (nt - 1) % 2  # the field the last application wrote
+    c = runs["f32"][0].compare(f, runs["f64"][0], f)
+    ratio = runs["f32"][1] / runs["f64"][1]
+    print("FP32 %.1f GLUP/s, FP64 %.1f GLUP/s, ratio %.3f, rel L2 %.3e, max |ref| %.3e"
+          % (runs["f32"][1], runs["f64"][1], ratio, c["rel_l2"], c["max_ref"]), flush=True)
+    for st, _ in runs.values():
+        st.close()
+    assert c["max_ref"] > 1e-3  # the wave has spread through the grid
+    assert c["rel_l2"] <= 1e-5
+    assert ratio >= 1.8

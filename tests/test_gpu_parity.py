@@ -93,13 +93,17 @@ RAGGED = {
 
 @pytest.mark.parametrize("kernel", [0, 1, 2, 3, 4, 5])
 @pytest.mark.parametrize("prec", ["dp", "sp"])
-def test_random_ragged_bitexact(kernel, prec):
+@pytest.mark.parametrize("strm", [0, 1])
+def test_random_ragged_bitexact(kernel, prec, strm):
+    """strm 1 forces the non-temporal kernel instantiations that problems larger than
+    the caches (the benchmark sizes) run, at sizes the oracle checks in seconds."""
     dt = np.float64 if prec == "dp" else np.float32
     dtype = gpu.GR_F64 if prec == "dp" else gpu.GR_F32
     for n, (nx, ny, nz) in enumerate(RAGGED[orc.RADIUS[kernel]]):
-        print("kernel", kernel, prec, (nx, ny, nz), flush=True)
+        print("kernel", kernel, prec, strm, (nx, ny, nz), flush=True)
         u, v, roc2, coef = _random_problem(kernel, nx, ny, nz, dt, 100 * kernel + n)
         with gpu.Stepper(kernel, nx, ny, nz, dtype=dtype, coef=_consts(coef, kernel)) as st:
+            st.set_streaming(strm)
             st.upload(_pad(u), _pad(v), _pad(roc2) if roc2 is not None else None, _pad_coef(coef, kernel))
             st.step(5)  # odd count: ends with U1 written
             U1, U2 = st.download(_pad(u).shape)
@@ -125,9 +129,11 @@ def _num_variants(st):
 
 @pytest.mark.parametrize("kernel", [0, 1, 5])
 @pytest.mark.parametrize("prec", ["dp", "sp"])
-def test_every_tile_variant_bitexact(kernel, prec):
+@pytest.mark.parametrize("strm", [0, 1])
+def test_every_tile_variant_bitexact(kernel, prec, strm):
     """Every launch configuration the autotuner may pick (tile variant, XCD order, z
-    chunk) gives the oracle's result bit for bit on a ragged grid."""
+    chunk, normal or non-temporal accesses) gives the oracle's result bit for bit on a
+    ragged grid."""
     dt = np.float64 if prec == "dp" else np.float32
     dtype = gpu.GR_F64 if prec == "dp" else gpu.GR_F32
     nx, ny, nz = (290, 37, 13) if orc.RADIUS[kernel] == 4 else (300, 45, 11)
@@ -138,9 +144,10 @@ def test_every_tile_variant_bitexact(kernel, prec):
     with gpu.Stepper(kernel, nx, ny, nz, dtype=dtype, coef=_consts(coef, kernel)) as st:
         nvar = _num_variants(st)
         assert nvar == (16 if kernel in (0, 1, 5) else 1)
+        st.set_streaming(strm)
         for var in range(nvar):
             for order, zc in ((0, 0), (1, 5)):
-                print("variant", kernel, prec, var, order, zc, flush=True)
+                print("variant", kernel, prec, strm, var, order, zc, flush=True)
                 st.upload(_pad(u), _pad(v), _pad(roc2) if roc2 is not None else None, _pad_coef(coef, kernel))
                 st.set_config(var, zc, 1)
                 st.set_xcd_order(order)
@@ -149,6 +156,53 @@ def test_every_tile_variant_bitexact(kernel, prec):
                 nnx = u.shape[2]
                 np.testing.assert_array_equal(U1[:, :, :nnx], uo, err_msg="variant %d" % var)
                 np.testing.assert_array_equal(U2[:, :, :nnx], vo, err_msg="variant %d" % var)
+
+
+WAVE_W = [-205.0 / 72, 8.0 / 5, -1.0 / 5, 8.0 / 315, -1.0 / 560]
+WAVE_COEF = [3 * WAVE_W[0]] + WAVE_W[1:5]
+
+
+@pytest.mark.parametrize("roc2_kind", ["scalar", "field", "bf16"])
+@pytest.mark.parametrize("strm", [0, 1])
+def test_fast_every_variant_bitexact_vs_oracle(roc2_kind, strm):
+    """The GR_FAST FP32 kernel-0 sweeps -- the benchmark's arithmetic -- for every tile
+    variant, workgroup order, z chunk and cache policy, with a point source, equal the
+    oracle's restatement of the GR_FAST arithmetic bit for bit.  strm 1 with the scalar
+    roc2 is the instantiation the 1024^3 benchmark times."""
+    nx, ny, nz = 290, 37, 13
+    kernel = 0
+    rng = np.random.default_rng(4242)
+    shp = orc.shape_of(kernel, nx, ny, nz)
+    u = np.zeros(shp, np.float32)
+    v = np.zeros(shp, np.float32)
+    u[4:-4, 4:-4, 4:-4] = rng.uniform(-1, 1, (nz, ny, nx))
+    v[4:-4, 4:-4, 4:-4] = rng.uniform(-1, 1, (nz, ny, nx))
+    roc2 = None
+    flags = gpu.GR_FAST
+    if roc2_kind == "scalar":
+        flags |= gpu.GR_ROC2_SCALAR
+    else:
+        roc2 = rng.uniform(0.05, 0.2, shp).astype(np.float32)
+        if roc2_kind == "bf16":
+            roc2 = _bf16_round(roc2)
+            flags |= gpu.GR_COEF_BF16
+    src = (np.cos(np.arange(4) * 1.3) * 0.5).astype(np.float64)
+    pt = (4 + 131, 4 + 17, 4 + 6)
+    uo, vo = u.copy(), v.copy()
+    orc.run_fast(WAVE_COEF, uo, vo, roc2=roc2, roc2s=0.13, nt=3, src_idx=pt, src=src.astype(np.float32))
+    with gpu.Stepper(kernel, nx, ny, nz, dtype=gpu.GR_F32, flags=flags, roc2=0.13, coef=WAVE_COEF) as st:
+        st.set_streaming(strm)
+        for var in range(_num_variants(st)):
+            for order, zc in ((0, 0), (1, 5)):
+                print("fast variant", roc2_kind, strm, var, order, zc, flush=True)
+                st.upload(_pad(u), _pad(v), _pad(roc2) if roc2 is not None else None)
+                st.set_source(*pt, src)
+                st.set_config(var, zc, 1)
+                st.set_xcd_order(order)
+                st.step(4)
+                U1, U2 = st.download(_pad(u).shape)
+                np.testing.assert_array_equal(U1[:, :, :shp[2]], uo, err_msg="variant %d" % var)
+                np.testing.assert_array_equal(U2[:, :, :shp[2]], vo, err_msg="variant %d" % var)
 
 
 def test_every_tile_variant_fast_identical():

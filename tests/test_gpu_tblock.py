@@ -23,14 +23,20 @@ SIZES = {1: [(70, 38, 20), (130, 9, 11), (64, 32, 8)], 4: [(72, 20, 17), (67, 33
 
 @pytest.mark.parametrize("kernel,depth", [(1, 2), (1, 3), (1, 4), (1, 5), (1, 6), (5, 2), (5, 4), (5, 6), (0, 2)])
 @pytest.mark.parametrize("prec", ["dp", "sp"])
-def test_fused_bitexact_vs_oracle(kernel, depth, prec):
+@pytest.mark.parametrize("strm", [0, 1])
+def test_fused_bitexact_vs_oracle(kernel, depth, prec, strm):
+    """strm 1 forces the non-temporal instantiations of the kernel-0 and kernel-1 passes
+    (the ones problems larger than the caches run; kernel 5's pass has one form)."""
+    if strm and kernel == 5:
+        pytest.skip("the kernel-5 pass has no non-temporal form")
     dt = np.float64 if prec == "dp" else np.float32
     dtype = gpu.GR_F64 if prec == "dp" else gpu.GR_F32
     for n, (nx, ny, nz) in enumerate(SIZES[orc.RADIUS[kernel]]):
         for napply in (2 * depth + 1, 3 * depth):
-            print("fused", kernel, depth, prec, (nx, ny, nz), napply, flush=True)
+            print("fused", kernel, depth, prec, strm, (nx, ny, nz), napply, flush=True)
             u, v, roc2, coef = _random_problem(kernel, nx, ny, nz, dt, 1000 + 10 * kernel + n)
             with gpu.Stepper(kernel, nx, ny, nz, dtype=dtype, coef=_consts(coef, kernel)) as st:
+                st.set_streaming(strm)
                 st.upload(_pad(u), _pad(v), _pad(roc2) if roc2 is not None else None, _pad_coef(coef, kernel))
                 st.set_config(t_block=depth)
                 st.step(napply)
@@ -94,17 +100,19 @@ def test_bf16_roc2_wave_keeps_single_sweeps():
     np.testing.assert_array_equal(U2[:, :, :nnx], v)
 
 
-@pytest.mark.parametrize("kernel,depth", [(1, 2), (1, 3), (1, 5), (1, 6), (5, 4), (0, 2)])
+@pytest.mark.parametrize("kernel,depth", [(1, 2), (1, 3), (1, 5), (1, 6), (5, 2), (5, 4), (0, 2)])
 @pytest.mark.parametrize("zchunk", [1, 5, 8])
-def test_fused_zchunks_bitexact_vs_oracle(kernel, depth, zchunk):
+@pytest.mark.parametrize("prec", ["dp", "sp"])
+def test_fused_zchunks_bitexact_vs_oracle(kernel, depth, zchunk, prec):
     """Workgroups that own a z chunk recompute the chunk's z ghost planes; the
     result must not depend on the chunk size (including chunks of one plane)."""
     nx, ny, nz = SIZES[orc.RADIUS[kernel]][0]
     napply = 2 * depth + (0 if kernel == 0 else 1)
-    print("fused zchunk", kernel, depth, zchunk, flush=True)
-    u, v, roc2, coef = _random_problem(kernel, nx, ny, nz, np.float64, 500 + kernel)
+    print("fused zchunk", kernel, depth, zchunk, prec, flush=True)
+    dt, dtype = (np.float64, gpu.GR_F64) if prec == "dp" else (np.float32, gpu.GR_F32)
+    u, v, roc2, coef = _random_problem(kernel, nx, ny, nz, dt, 500 + kernel)
     outs = []
-    with gpu.Stepper(kernel, nx, ny, nz, dtype=gpu.GR_F64, coef=_consts(coef, kernel)) as st:
+    with gpu.Stepper(kernel, nx, ny, nz, dtype=dtype, coef=_consts(coef, kernel)) as st:
         for order in (0, 1):  # plain and XCD-contiguous workgroup order
             st.upload(_pad(u), _pad(v), _pad(roc2) if roc2 is not None else None, _pad_coef(coef, kernel))
             st.set_config(zchunk=zchunk, t_block=depth)
@@ -177,6 +185,39 @@ def test_fused_wave_tiles_and_source_on_tile_edges(prec, roc2_kind):
     orc.run(kernel, coef, u, v, roc2, nt=10, src_idx=pt, src=src)
     np.testing.assert_array_equal(U1[:, :, :u.shape[2]], u)
     np.testing.assert_array_equal(U2[:, :, :u.shape[2]], v)
+
+
+@pytest.mark.parametrize("strm", [0, 1])
+@pytest.mark.parametrize("zchunk", [0, 3, 7])
+def test_fused_fast_bitexact_vs_oracle(strm, zchunk):
+    """GR_FAST two-step wave passes with a scalar roc2 (the benchmark's arithmetic and
+    fused form), over several tiles and z chunks, with a point source on a tile edge:
+    bit-identical to the oracle's restatement of the GR_FAST arithmetic."""
+    nx, ny, nz = 200, 37, 23
+    kernel = 0
+    rng = np.random.default_rng(90 + zchunk)
+    shp = orc.shape_of(kernel, nx, ny, nz)
+    u = np.zeros(shp, np.float32)
+    v = np.zeros(shp, np.float32)
+    u[4:-4, 4:-4, 4:-4] = rng.uniform(-1, 1, (nz, ny, nx))
+    v[4:-4, 4:-4, 4:-4] = rng.uniform(-1, 1, (nz, ny, nx))
+    w = [-205.0 / 72, 8.0 / 5, -1.0 / 5, 8.0 / 315, -1.0 / 560]
+    coef = [3 * w[0]] + w[1:5]
+    src = np.linspace(1.0, -1.0, 10)
+    pt = (4 + 64, 4 + 16, 4 + 9)
+    print("fused fast", strm, zchunk, flush=True)
+    with gpu.Stepper(kernel, nx, ny, nz, dtype=gpu.GR_F32, flags=gpu.GR_FAST | gpu.GR_ROC2_SCALAR, roc2=0.12,
+                     coef=coef) as st:
+        st.set_streaming(strm)
+        st.upload(_pad(u), _pad(v))
+        st.set_source(*pt, src)
+        st.set_config(zchunk=zchunk, t_block=2)
+        st.step(10)
+        assert st.stats()["last_launches"] == 5
+        U1, U2 = st.download(_pad(u).shape)
+    orc.run_fast(coef, u, v, roc2s=0.12, nt=10, src_idx=pt, src=src.astype(np.float32))
+    np.testing.assert_array_equal(U1[:, :, :shp[2]], u)
+    np.testing.assert_array_equal(U2[:, :, :shp[2]], v)
 
 
 def test_fused_fast_fp32_multi_tile_equals_single_sweeps():
