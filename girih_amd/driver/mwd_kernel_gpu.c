@@ -199,6 +199,13 @@ static void init(Params *p) {
       die("Relaxed synchronization implementations are disabled for stencil radius > 1 for performance reasons");
     if (p->t_dim == -1) p->t_dim = 1; /* girih autotunes t_dim; the GPU does not need it */
     if (p->thread_group_size == 1) p->thx = p->thy = p->thz = p->thc = 1;
+    if (p->num_wf == -1) p->num_wf = 1;
+    /* diamond_utils.c:1042-1056: nt rounded (and reported) at initialisation */
+    const int nt2 = gr_mwd_round_nt(p->nt, p->t_dim);
+    if (nt2 != p->nt) {
+      if (p->verbose) printf("###INFO: Modified nt from %03d to %03d for the intra-diamond method to work properly\n", p->nt, nt2);
+      p->nt = nt2;
+    }
   }
 }
 
@@ -309,9 +316,10 @@ static void gpu_ts(Params *p, gr_handle *h) {
   p->nt_applied = nt_applied;
 }
 
-static void print_param(const Params *p) {
-  gr_stats s;
-  memset(&s, 0, sizeof(s));
+/* utils.c:1012-1065.  The keys and their order are girih's; values that describe
+ * CPU threading or cache blocking, which the GPU stepper does not have, print as
+ * girih's single-thread values (threads 1) or 0 (cache block sizes). */
+static void print_param(const Params *p, const gr_stats *s) {
   printf("\n******************************************************\n");
   printf("Parameters settings\n");
   printf("******************************************************\n");
@@ -333,14 +341,32 @@ static void print_param(const Params *p) {
   printf("Using separate call to central line update: %d\n", 0);
   printf("Halo concatenation: %d\n", p->halo_concat);
   if (p->target_ts == 2) {
+    const int tgs = p->thread_group_size > 0 ? p->thread_group_size : 1;
     printf("Enable wavefronts: %d\n", 1);
     printf("Wavefront parallel strategy: %s\n", MWD_NAME[p->mwd_type >= 0 && p->mwd_type < 4 ? p->mwd_type : 0]);
     printf("Intra-diamond width:   %d\n", (p->t_dim + 1) * 2 * p->R);
+    printf("Wavefront width:  %d\n", p->t_dim * 2 * p->R + p->num_wf);
+    printf("Cache block size/wf (kiB): %d\n", 0);
+    printf("Total cache block size (kiB): %d\n", 0);
+    printf("Next larger cache block size/wf (kiB): %d (diam_width=%d)\n", 0, (p->t_dim + 3) * 2 * p->R);
     printf("Intra-diamond prologue/epilogue MStencils: %d\n", 0);
-    printf("Thread group size: %d\n", p->thread_group_size);
+    printf("Multi-wavefront updates: %d\n", p->num_wf);
+    printf("User set thread group size: %d\n", p->thread_group_size);
+    printf("Thread group size: %d\n", tgs);
+    printf("Threads along z-axis: %d\n", p->thz > 0 ? p->thz : tgs);
+    printf("Threads along y-axis: %d\n", p->thy > 0 ? p->thy : 1);
+    printf("Threads along x-axis: %d\n", p->thx > 0 ? p->thx : 1);
+    printf("Threads per cell    : %d\n", p->thc > 0 ? p->thc : 1);
+    printf("Threads block: %d\n", 1);
+    printf("Threads stride: %d\n", 1);
+  } else {
+    printf("Block size in Y: %d\n", s->tile_y);
+    printf("OpenMP schedule: %s\n", "none (GPU workgroups)");
   }
   printf("GPU arithmetic: %s\n", p->fast ? "FP32 compensated (--fast)" : "girih operand order");
+  printf("GPU tile: %dx%d  z-chunk: %d  fused steps: %d\n", s->tile_x, s->tile_y, s->zchunk, s->t_block);
   printf("OpenMP Threads: %d\n", 1);
+  printf("Assumed usable cache size: %dKiB\n", p->cache_size);
   printf("MPI size: %d\n", 1);
   printf("Processors topology (npx, npy, npz): 01,01,01\n");
   printf("******************************************************\n");
@@ -487,11 +513,15 @@ static int cmp_d(const void *a, const void *b) {
 static void performance_test(Params *p) {
   time_t t0 = time(NULL);
   printf("Started on %s", ctime(&t0));
-  if (p->verbose) print_param(p);
   arrays_allocate(p);
   init_coeff(p);
   domain_data_fill(p);
-  gr_handle *h = gpu_setup(p);
+  gr_handle *h = gpu_setup(p); /* after autotuning (ts 3) the launch configuration is known */
+  if (p->verbose) {
+    gr_stats s0;
+    check(gr_get_stats(h, &s0), "gr_get_stats");
+    print_param(p, &s0);
+  }
   printf("\n******************************************************\nPerformance results\n"
          "******************************************************\n");
   double *tt = calloc(p->n_tests > 0 ? p->n_tests : 1, sizeof(double));
@@ -505,6 +535,8 @@ static void performance_test(Params *p) {
     const double per = tt[i] / p->nt_applied;
     if (per < t_min) t_min = per;
     if (per > t_max) t_max = per;
+    printf("Rank 0 TEST#%02d time: %e\n", i + 1, tt[i]); /* performance.c:87-95 */
+    if (i == p->n_tests - 1) printf("******************************************************\n");
   }
   qsort(tt, p->n_tests, sizeof(double), cmp_d);
   const double t_med = p->n_tests > 0 ? tt[p->n_tests / 2] : 0;
@@ -541,15 +573,54 @@ static void performance_test(Params *p) {
     printf("RANK0 Communication: %f (s) - %05.2f%%\n", 0.0, 0.0);
     printf("RANK0 Waiting: %f (s) - %05.2f%%\n", 0.0, 0.0);
     printf("RANK0 Other: %f (s) - %05.2f%%\n", 0.0, 0.0);
+    /* one process: the MEAN / MAX / MIN over ranks equal rank 0's figures (utils.c:884-903) */
+    static const char *stat[3] = {"MEAN", "MAX", "MIN"};
+    for (int q = 0; q < 3; q++) {
+      printf("\n******************************************************\n");
+      if (q == 0) {
+        printf("MEAN Total: %f (s) -%06.2f%%\n", t, 100.0);
+        printf("MEAN Computation: %f (s) - %05.2f%%\n", t, 100.0);
+        printf("MEAN Communication: %f (s) - %05.2f%%\n", 0.0, 0.0);
+        printf("MEAN Waiting: %f (s) - %05.2f%%\n", 0.0, 0.0);
+        printf("MEAN Other: %f (s) - %05.2f%%\n", 0.0, 0.0);
+      } else {
+        printf("%s Total: %f (s)\n", stat[q], t);
+        printf("%s Computation: %f (s)\n", stat[q], t);
+        printf("%s Communication: %f (s)\n", stat[q], 0.0);
+        printf("%s Waiting: %f (s)\n", stat[q], 0.0);
+        printf("%s Other: %f (s)\n", stat[q], 0.0);
+      }
+    }
   } else {
     printf("\nTotal RANK0 MStencil/s MIN: %f  \n", p->ln_stencils / (1e6 * t_max));
     printf("Total RANK0 MStencil/s MAX: %f  \n", p->ln_stencils / (1e6 * t_min));
     printf("******************************************************\n");
     printf("MWD main-loop RANK0 MStencil/s MIN: %f\n", p->ln_stencils * (double)nt / 1e6 / (t_max * nt));
     printf("MWD main-loop RANK0 MStencil/s MAX: %f\n", p->ln_stencils * (double)nt / 1e6 / (t_min * nt));
+    printf("******************************************************\n");
+    /* utils.c:908-980: one GPU stands for girih's single core and thread group; the
+     * whole run is the diamond main loop (no prologue, synchronisation or spin waits) */
     printf("%-27s %f (s) - %05.2f%%\n", "RANK0 ts main loop:", t, 100.0);
+    printf("%-27s %f (s) - %05.2f%%\n", "RANK0 ts prologue/epilogue:", 0.0, 0.0);
+    printf("%-27s %f (s) - %05.2f%%\n", "RANK0 ts others:", 0.0, 0.0);
+    printf("******************************************************\n");
+    printf("%-30s  core %02d     \n", "Metric \\ core:", 0);
+    printf("%-27s  %e\n", "Wavefront synchronization [s]:", 0.0);
+    printf("%-27s  %05.2f       \n\n", "Wavefront synchronization [%]:", 0.0);
+    printf("%-27s  group %02d    \n", "Metric \\ thread group:", 0);
+    printf("%-27s  %e\n", "Wavefront steady state [s]:", t);
+    printf("%-27s  %05.2f       \n", "Wavefront steady state [%]:", 100.0);
+    printf("%-27s  %e\n", "Wavefront startup/end [s]:", 0.0);
+    printf("%-27s  %05.2f       \n", "Wavefront startup/end [%]:", 0.0);
+    printf("%-27s  %e\n", "Wavefront communication [s]:", 0.0);
+    printf("%-27s  %05.2f       \n", "Wavefront communication [%]:", 0.0);
+    printf("%-27s  %e\n", "Wavefront others [s]:", 0.0);
+    printf("%-27s  %05.2f       \n", "Wavefront others [%]:", 0.0);
+    printf("%-27s  %e\n", "Group spin-wait [s]:", 0.0);
+    printf("%-27s  %05.2f       \n", "Group spin-wait [%]:", 0.0);
+    printf("%-27s  %e", "Resolved diamonds:", 0.0);
   }
-  printf("GPU tile: %dx%d  z-chunk: %d  fused steps: %d  device time of last test (ms): %f\n", s.tile_x, s.tile_y,
+  printf("\nGPU tile: %dx%d  z-chunk: %d  fused steps: %d  device time of last test (ms): %f\n", s.tile_x, s.tile_y,
          s.zchunk, s.t_block, s.last_ms);
   printf("\n******************************************************\n");
   free(tt);

@@ -84,11 +84,64 @@ def test_performance_report_keys(ts):
             "--t-dim", 1, "--n-tests", 3)
     assert r.returncode == 0, r.stdout + r.stderr
     out = r.stdout
+    nt = 22 if ts == 2 else 20  # the diamond stepper rounds nt at initialisation (diamond_utils.c:1042-1056)
     for key in ("Time stepper name:", "Stencil Kernel name: star", "Stencil Kernel semi-bandwidth: 4",
-                "Precision: SP", "Number of time steps: 20", "Number of tests: 3", "COMPLETED SUCCESSFULLY"):
+                "Precision: SP", "Number of time steps: %d" % nt, "Number of tests: 3", "COMPLETED SUCCESSFULLY"):
         assert key in out, key
+    if ts == 2:
+        assert "###INFO: Modified nt from 020 to 022" in out
     if ts == 2:
         assert re.search(r"MWD main-loop RANK0 MStencil/s MAX: [0-9.]+", out)
         assert "Intra-diamond width:   16" in out
     else:
         assert re.search(r"RANK0 GStencil/s MEDIAN: [0-9.]+", out)
+
+
+REF_BIN = os.path.join(REPO, "oracle", "_ref", "mwd_kernel_sp")
+
+
+def _report_keys(text):
+    """The 'Key' of every 'Key: value' line of a girih report (what scripts/parse.py
+    matches on), with the time stamps of the first and last lines dropped."""
+    keys = set()
+    for line in text.splitlines():
+        if ":" not in line or line.startswith(("libgomp", "[AUTO TUNE]")):
+            continue
+        key = line.split(":", 1)[0].rstrip()
+        for stamp in ("Started on", "COMPLETED SUCCESSFULLY on"):
+            if key.startswith(stamp):
+                key = stamp
+        keys.add(key)
+    return keys
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", [
+    ["--target-ts", 0, "--target-kernel", 1, "--nx", 64, "--ny", 32, "--nz", 32, "--nt", 10,
+     "--thread-group-size", 1],
+    ["--target-ts", 2, "--target-kernel", 0, "--nx", 32, "--ny", 64, "--nz", 32, "--nt", 10, "--t-dim", 1,
+     "--thread-group-size", 1, "--thx", 1, "--thy", 1, "--thz", 1, "--num-wavefronts", 1],
+    ["--target-ts", 2, "--target-kernel", 5, "--nx", 32, "--ny", 64, "--nz", 32, "--nt", 10, "--t-dim", 3,
+     "--thread-group-size", 2, "--thx", 1, "--thy", 1, "--thz", 2, "--num-wavefronts", 2, "--mwd-type", 1],
+])
+def test_report_has_every_key_of_girihs_report(case):
+    """Every 'Key:' line girih's own mwd_kernel prints for these flags (built from its
+    sources into oracle/_ref) also appears in the GPU driver's report, so that
+    scripts/parse.py reads GPU runs like CPU runs (utils.c:866-1065)."""
+    if not os.path.exists(REF_BIN):
+        pytest.skip("girih's mwd_kernel is not built (oracle/Makefile ref)")
+    args = [str(a) for a in case] + ["--n-tests", "3"]
+    ref = subprocess.run([REF_BIN] + args, capture_output=True, text=True, timeout=120,
+                         env=dict(os.environ, OMP_NUM_THREADS="2"))
+    assert ref.returncode == 0, ref.stdout[-2000:] + ref.stderr[-2000:]
+    r = run("sp", *args)
+    assert r.returncode == 0, r.stdout + r.stderr
+    missing = _report_keys(ref.stdout) - _report_keys(r.stdout)
+    assert not missing, sorted(missing)
+    # the values parse.py compares across runs agree where they describe the problem
+    for key in ("Number of time steps", "Time unroll", "Intra-diamond width", "Stencil Kernel semi-bandwidth",
+                "Precision", "Multi-wavefront updates", "Wavefront width"):
+        ref_v = re.search(r"^%s:\s*(.*)$" % re.escape(key), ref.stdout, re.M)
+        gpu_v = re.search(r"^%s:\s*(.*)$" % re.escape(key), r.stdout, re.M)
+        if ref_v:
+            assert gpu_v and gpu_v.group(1).strip() == ref_v.group(1).strip(), key
