@@ -17,6 +17,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1117,23 +1118,36 @@ hipError_t launch_compare(const gr::Problem &A, const void *a, const gr::Problem
 
 }  // namespace
 
-extern "C" int gr_compare_fields(gr_handle *a, int fa, gr_handle *b, int fb, double out[4]) {
-  if (!a || !b || !out) return fail(GR_EINVAL, "gr_compare_fields: null argument");
-  if (fa < 0 || fa > 1 || fb < 0 || fb > 1) return fail(GR_EINVAL, "gr_compare_fields: field must be 0 (U1) or 1 (U2)");
+// Partial sums per workgroup of the comparison (compare_kernel's grid).
+constexpr int kComparePartials = 2048;
+
+static int compare_args(gr_handle *a, int fa, gr_handle *b, int fb, const char *who) {
+  if (!a || !b) return fail(GR_EINVAL, "%s: null argument", who);
+  if (fa < 0 || fa > 1 || fb < 0 || fb > 1) return fail(GR_EINVAL, "%s: field must be 0 (U1) or 1 (U2)", who);
   const Problem &A = a->P, &B = b->P;
-  if (A.nx != B.nx || A.ny != B.ny || A.nz != B.nz || A.R != B.R)
-    return fail(GR_EINVAL, "gr_compare_fields: grids differ");
+  if (A.nx != B.nx || A.ny != B.ny || A.nz != B.nz || A.R != B.R) return fail(GR_EINVAL, "%s: grids differ", who);
+  return GR_OK;
+}
+
+static hipError_t launch_compare_any(gr_handle *a, int fa, gr_handle *b, int fb, double *part, int nblk) {
+  const Problem &A = a->P, &B = b->P;
   const void *pa = ubase(a, fa);
   const void *pb = ubase(b, fb);
-  const int nblk = 2048;
+  if (A.dtype == GR_F64 && B.dtype == GR_F64) return launch_compare<double, double>(A, pa, B, pb, part, nblk, a->stream);
+  if (A.dtype == GR_F32 && B.dtype == GR_F64) return launch_compare<float, double>(A, pa, B, pb, part, nblk, a->stream);
+  if (A.dtype == GR_F64 && B.dtype == GR_F32) return launch_compare<double, float>(A, pa, B, pb, part, nblk, a->stream);
+  return launch_compare<float, float>(A, pa, B, pb, part, nblk, a->stream);
+}
+
+extern "C" int gr_compare_fields(gr_handle *a, int fa, gr_handle *b, int fb, double out[4]) {
+  int rc = compare_args(a, fa, b, fb, "gr_compare_fields");
+  if (rc) return rc;
+  if (!out) return fail(GR_EINVAL, "gr_compare_fields: null argument");
+  const int nblk = kComparePartials;
   double *part = nullptr;
   HIP_TRY(hipMalloc(&part, sizeof(double) * 4 * nblk));
   HIP_TRY(hipStreamSynchronize(b->stream));
-  hipError_t e;
-  if (A.dtype == GR_F64 && B.dtype == GR_F64) e = launch_compare<double, double>(A, pa, B, pb, part, nblk, a->stream);
-  else if (A.dtype == GR_F32 && B.dtype == GR_F64) e = launch_compare<float, double>(A, pa, B, pb, part, nblk, a->stream);
-  else if (A.dtype == GR_F64 && B.dtype == GR_F32) e = launch_compare<double, float>(A, pa, B, pb, part, nblk, a->stream);
-  else e = launch_compare<float, float>(A, pa, B, pb, part, nblk, a->stream);
+  hipError_t e = launch_compare_any(a, fa, b, fb, part, nblk);
   if (e != hipSuccess) {
     (void)hipFree(part);
     return fail(GR_EHIP, "compare launch: %s", hipGetErrorString(e));
@@ -1150,6 +1164,86 @@ extern "C" int gr_compare_fields(gr_handle *a, int fa, gr_handle *b, int fb, dou
     out[2] = std::max(out[2], h[4 * i + 2]);
     out[3] = std::max(out[3], h[4 * i + 3]);
   }
+  return GR_OK;
+}
+
+// ------------------------------------------------------------------ comparison without allocation
+// gr_compare_fields_static keeps the partial sums in `scratch`, one copy per GPU
+// (a file-scope device array: the library holds it on every GPU it runs on until
+// the process ends), and reduces them on the device in gr_compare_fields's order.
+// Host code never names `scratch` as device memory: it uses the address that
+// hipGetSymbolAddress returned for the current device.  Every call writes all
+// partials before reading them, so nothing needs clearing between calls; calls
+// that overlap on one GPU share the array and race.
+namespace {
+
+constexpr int kScratchDoubles = 4 * kComparePartials + 4;  // partials, then the 4 results
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[kMaxDevices] = {};
+
+// The reduction of the host loop of gr_compare_fields, in the same order.
+__global__ void compare_finish_kernel(const double *part, int nblk, double *res) {
+  if (threadIdx.x != 0) return;
+  double o0 = 0.0, o1 = 0.0, o2 = 0.0, o3 = 0.0;
+  for (int i = 0; i < nblk; ++i) {
+    o0 += part[4 * i];
+    o1 += part[4 * i + 1];
+    o2 = (o2 < part[4 * i + 2]) ? part[4 * i + 2] : o2;
+    o3 = (o3 < part[4 * i + 3]) ? part[4 * i + 3] : o3;
+  }
+  res[0] = o0;
+  res[1] = o1;
+  res[2] = o2;
+  res[3] = o3;
+}
+
+// This device's copy of `scratch` (looked up once per device, with it current).
+int scratch_ptr(int dev, double **out) {
+  if (dev < 0 || dev >= kMaxDevices) return fail(GR_EUNSUPPORTED, "gr_compare_fields_static: device %d beyond %d", dev, kMaxDevices);
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(GR_EHIP, "gr_compare_fields_static: hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+    g_scratch_ptr[dev] = (double *)p;
+  }
+  *out = g_scratch_ptr[dev];
+  return GR_OK;
+}
+
+}  // namespace
+
+extern "C" int gr_compare_fields_static(gr_handle *a, int fa, gr_handle *b, int fb, double out[4]) {
+  int rc = compare_args(a, fa, b, fb, "gr_compare_fields_static");
+  if (rc) return rc;
+  if (!out) return fail(GR_EINVAL, "gr_compare_fields_static: null argument");
+  const int nblk = kComparePartials;
+  if (4 * nblk + 4 > kScratchDoubles)
+    return fail(GR_EUNSUPPORTED, "gr_compare_fields_static: %d partials exceed the %d-double scratch array", nblk,
+                kScratchDoubles);
+  int cur = 0;
+  HIP_TRY(hipGetDevice(&cur));
+  if (cur != a->device) HIP_TRY(hipSetDevice(a->device));  // the handle's GPU is current for the call
+  double *base = nullptr;
+  rc = scratch_ptr(a->device, &base);
+  hipError_t e = hipSuccess;
+  if (rc == GR_OK) e = hipStreamSynchronize(b->stream);
+  if (rc == GR_OK && e == hipSuccess) e = launch_compare_any(a, fa, b, fb, base, nblk);
+  if (rc == GR_OK && e == hipSuccess) {
+    hipLaunchKernelGGL(compare_finish_kernel, dim3(1), dim3(64), 0, a->stream, (const double *)base, nblk,
+                       base + 4 * nblk);
+    e = hipGetLastError();
+  }
+  if (rc == GR_OK && e == hipSuccess)
+    e = hipMemcpyAsync(out, base + 4 * nblk, 4 * sizeof(double), hipMemcpyDeviceToHost, a->stream);
+  if (rc == GR_OK && e == hipSuccess) e = hipStreamSynchronize(a->stream);
+  if (cur != a->device) (void)hipSetDevice(cur);
+  if (rc) return rc;
+  if (e != hipSuccess) return fail(GR_EHIP, "gr_compare_fields_static: %s", hipGetErrorString(e));
   return GR_OK;
 }
 

@@ -67,7 +67,7 @@ class BatchStats(ctypes.Structure):
 
 EXPORTS = ["gr_create", "gr_destroy", "gr_last_error", "gr_upload", "gr_download", "gr_step", "gr_run_ts",
            "gr_set_source", "gr_get_stats", "gr_mwd_round_nt", "gr_sweep_dev", "gr_field_ptr", "gr_init_fields",
-           "gr_init_coef_uniform", "gr_compare_fields", "gr_set_config", "gr_set_xcd_order", "gr_set_streaming",
+           "gr_init_coef_uniform", "gr_compare_fields", "gr_compare_fields_static", "gr_set_config", "gr_set_xcd_order", "gr_set_streaming",
            "gr_set_graph", "gr_autotune",
            "gr_init_roc2_model",
            "gr_download_roc2", "gr_batch_create", "gr_batch_destroy", "gr_batch_problem", "gr_batch_set_config",
@@ -109,6 +109,7 @@ def lib():
             "gr_init_coef_uniform": ([P, ctypes.c_uint64, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                       ctypes.c_double], I),
             "gr_compare_fields": ([P, I, P, I, ctypes.POINTER(ctypes.c_double)], I),
+            "gr_compare_fields_static": ([P, I, P, I, ctypes.POINTER(ctypes.c_double)], I),
             "gr_set_config": ([P, I, I, I], I),
             "gr_set_xcd_order": ([P, I], I),
             "gr_set_streaming": ([P, I], I),
@@ -275,6 +276,13 @@ class Stepper:
         """FP64 device reduction: {sum (a-b)^2, sum b^2, max|a-b|, max|b|} over the interior."""
         out = (ctypes.c_double * 4)()
         check(lib().gr_compare_fields(self._h, int(field), other._h, int(other_field), out))
+        return {"sq_diff": out[0], "sq_ref": out[1], "max_diff": out[2], "max_ref": out[3],
+                "rel_l2": (out[0] / out[1]) ** 0.5 if out[1] > 0 else float("nan")}
+
+    def compare_static(self, field, other, other_field):
+        """compare() through gr_compare_fields_static (no device allocation)."""
+        out = (ctypes.c_double * 4)()
+        check(lib().gr_compare_fields_static(self._h, int(field), other._h, int(other_field), out))
         return {"sq_diff": out[0], "sq_ref": out[1], "max_diff": out[2], "max_ref": out[3],
                 "rel_l2": (out[0] / out[1]) ** 0.5 if out[1] > 0 else float("nan")}
 

@@ -180,6 +180,14 @@ int gr_autotune(gr_handle *h, int steps, gr_stats *best);
  * precisions may differ).  out = { sum (a-b)^2, sum b^2, max |a-b|, max |b| }.
  * b may equal a (then fa != fb compares the two fields of one handle). */
 int gr_compare_fields(gr_handle *a, int fa, gr_handle *b, int fb, double out[4]);
+/* gr_compare_fields without device allocation: no counterpart in girih; for callers
+ * that cannot call hipMalloc (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid it).  Same arguments, checks and results, bit for bit.  The
+ * 2048 x 4 FP64 partial sums and the 4 results live in a static device array of
+ * 8196 doubles (64 KiB) that the library holds on every GPU it runs on until the
+ * process ends; the partial count does not depend on the grid size.  All calls on
+ * one GPU share that array: calls that overlap on one GPU race. */
+int gr_compare_fields_static(gr_handle *a, int fa, gr_handle *b, int fb, double out[4]);
 
 /* ---- Velocity models for kernel 0 (roc2 = (v*dt/h)^2, girih's U3 array) ----
  * No counterpart in girih, which fills U3 with its closed-form ramp
