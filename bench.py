@@ -358,7 +358,7 @@ def extra_adjoint(n=400, nt=2000, ckpt=50, nfd=20, seed=7):
         rel = float(torch.linalg.norm(g32 - g64) / torch.linalg.norm(g64))
         J0 = float(0.5 * ((tr64 - d_obs) ** 2).sum())
         # the FP32 gradient error follows from the FP32 traces' error relative to the residual
-        tr_err = float(torch.linalg.norm(tr32.double() - tr64) / torch.linalg.norm(tr64))
+        tr_err = float(torch.linalg.norm(tr32.detach().double() - tr64) / torch.linalg.norm(tr64))
         res_rel = float(torch.linalg.norm(tr64 - d_obs) / torch.linalg.norm(tr64))
         del tr32, tr64
         g = g64.cpu().numpy()
