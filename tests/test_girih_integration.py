@@ -77,3 +77,36 @@ def test_girih_rejects_bad_diamond_width_before_gpu():
                                        "--nt", 9, "--t-dim", 1, "--verify", 1] + DIAMOND)
     assert r.returncode != 0
     assert "diamond width" in (r.stdout + r.stderr)
+
+
+# girih's regression matrix (scripts/verification), one rank: verification_std.py's
+# grids x kernels 0-5 for the spatial-blocking semantics, and the grids and diamond
+# depths of verification_idiam.py's general part for the diamond semantics, in both
+# precisions, with girih's nt = max(10, nx/R/2) (verification_utils.py:14).
+RADIUS = {0: 4, 1: 1, 2: 1, 3: 1, 4: 4, 5: 1}
+STD_GRIDS = [(16, 32, 64), (32, 64, 16), (64, 16, 32)]
+IDIAM_GRIDS = [(16, 32, 128, 1), (16, 64, 128, 3), (16, 64, 128, 7), (32, 128, 128, 7)]
+
+
+def _nt(nx, kernel):
+    return max(10, nx // RADIUS[kernel] // 2)
+
+
+@pytest.mark.parametrize("prec", ["sp", "dp"])
+@pytest.mark.parametrize("kernel", [0, 1, 2, 3, 4, 5])
+def test_girih_verification_std_matrix(prec, kernel):
+    for nx, ny, nz in STD_GRIDS:
+        r = _run("mwd_kernel_gputs_" + prec, ["--target-ts", 3, "--target-kernel", kernel, "--nx", nx, "--ny", ny,
+                                             "--nz", nz, "--nt", _nt(nx, kernel), "--verbose", 0, "--verify", 1])
+        assert r.returncode == 0 and PASS.search(r.stdout), ((nx, ny, nz), r.stdout[-1500:] + r.stderr[-1500:])
+
+
+@pytest.mark.parametrize("prec", ["sp", "dp"])
+@pytest.mark.parametrize("kernel", [0, 1, 2, 3, 4, 5])
+def test_girih_verification_idiam_matrix(prec, kernel):
+    for nx, ny, nz, t_dim in IDIAM_GRIDS:
+        r = _run("mwd_kernel_gpudiam_" + prec, ["--target-ts", 2, "--target-kernel", kernel, "--nx", nx, "--ny", ny,
+                                               "--nz", nz, "--nt", _nt(nx, kernel), "--t-dim", t_dim, "--verbose", 0,
+                                               "--verify", 1, "--thread-group-size", 1, "--thx", 1, "--thy", 1,
+                                               "--thz", 1, "--num-wavefronts", 4, "--mwd-type", 1])
+        assert r.returncode == 0 and PASS.search(r.stdout), ((nx, ny, nz, t_dim), r.stdout[-1500:] + r.stderr[-1500:])
