@@ -144,9 +144,14 @@ def cpu_baseline(n, nt_c3, full=False):
     try:
         def tuned(prec, kernel, shape, nt, tune_shape, tune_nt):
             cands = feasible_t_dims(kernel, shape[1], min(shape[2], tune_shape[2]), threads)
-            trials = {}
+            trials, errors = {}, []
             for t in cands:  # tuning: one test each on a reduced z extent (same diamonds and wavefronts)
-                trials[t] = _girih(prec, kernel, tune_shape, tune_nt, 2, t, threads, 1)[0]
+                try:
+                    trials[t] = _girih(prec, kernel, tune_shape, tune_nt, 2, t, threads, 1)[0]
+                except RuntimeError as e:  # a depth girih rejects for this grid is skipped
+                    errors.append(str(e))
+            if not trials:
+                raise RuntimeError("no feasible diamond depth: %s" % errors)
             best = max(trials, key=trials.get)
             gl, nt_run, wall = _girih(prec, kernel, shape, nt, 2, best, threads, 3)
             return {"glups": round(gl, 3), "t_dim": best, "tuning_glups": {str(k): round(v, 3) for k, v in trials.items()},
