@@ -309,8 +309,12 @@ def extra_batch(nt, seed=2024):
     return {"glups": round(gl, 2), "ms_per_step": round(s["last_ms"] / nt, 4), "problems": len(sizes),
             "cells": cells, "steps": nt, "sizes": sizes, "items_per_step": s["items"],
             "persistent_workgroups": s["grid"], "tile": [s["tile_x"], s["tile_y"]], "zchunk": s["zchunk"],
+            "time_block": s["time_block"], "block_variant": s["block_variant"], "launches": s["last_launches"],
             "graph_first_call_ms": round(t_first, 2),
-            "sweep_roofline_frac": round(gl * 14 / HBM_PEAK_GBS, 4), "bytes_per_lup_sweep": 14,
+            # algorithmic bytes per update: single steps read u^(n-1), u^n and roc2 (bf16) and write
+            # u^(n+1): 14 B; a two-step block reads 4 + 4 + 2 and writes 4 + 4 per two updates: 9 B
+            "roofline_frac": round(gl * (9 if s["time_block"] == 2 else 14) / HBM_PEAK_GBS, 4),
+            "bytes_per_lup": 9 if s["time_block"] == 2 else 14,
             "rel_l2_vs_f64_max": max(errs), "rel_l2_vs_f64": [float("%.3g" % e) for e in errs]}
 
 

@@ -29,6 +29,7 @@
 #include "../../include/girih_gpu.h"
 #include "gr_internal.hpp"
 #include "gr_sweep.hpp"
+#include "gr_wave2.hpp"
 
 namespace gr {
 namespace {
@@ -120,7 +121,70 @@ __global__ __launch_bounds__(64 * BX * BY) void batch_kernel(const BatchDesc *__
   }
 }
 
+// Two-step blocks: one item is (grid, xy tile, z chunk) for two time steps, run on
+// chip by the two-step pass's tile body (gr_wave2.hpp): it reads (u^(n-1), u^n) and
+// writes (u^(n+1), u^(n+2)) into the problem's spare pair.  `pass` counts the blocks
+// of the call; the block adds source values 2*pass and 2*pass + 1 of the call.
+template <typename C>
+struct W2BatchDesc {
+  W2Args<float, C> a;  // buffers of this block, tiles, z chunk, coefficients, source
+  int item0;           // first work item of this problem
+  int nsrc;            // source values available from a.src
+};
+static_assert(sizeof(W2BatchDesc<float>) % 8 == 0, "descriptor words");
+static_assert(sizeof(W2BatchDesc<float>) == sizeof(W2BatchDesc<__hip_bfloat16>), "one table size for both");
+
+using W2Tile = Wave2Tile<float, float, true, false, true, 1, 8, 2>;  // 64 x 16 output tiles, 512 lanes
+constexpr int kW2Lanes = W2Tile::NT;
+
+// MINW = 4 caps the registers at 128 so that two workgroups share a CU (a few
+// queue values spill); MINW = 1 keeps every value in registers, one workgroup per CU.
+template <typename C, int MINW>
+__global__ __launch_bounds__(kW2Lanes) __attribute__((amdgpu_waves_per_eu(MINW))) void batch_w2_kernel(
+    const W2BatchDesc<C> *__restrict__ d, int nprob, int nitems, unsigned long long *ctr, unsigned long long base,
+    int pass) {
+  using WT = Wave2Tile<float, C, true, false, true, 1, 8, 2>;  // a block streams every problem
+  __shared__ typename WT::Lds lds;
+  __shared__ int sclaim[2];
+  const bool leader = threadIdx.x == 0;
+  for (;;) {
+    if (leader) {
+      const int it = (int)(atomicAdd(ctr, 1ull) - base);
+      int p = 0;
+      if (it < nitems)
+        while (p + 1 < nprob && d[p + 1].item0 <= it) ++p;
+      sclaim[0] = it;
+      sclaim[1] = p;
+    }
+    __syncthreads();
+    const int it = __builtin_amdgcn_readfirstlane(sclaim[0]);
+    const int p = __builtin_amdgcn_readfirstlane(sclaim[1]);
+    if (it >= nitems) break;
+    const W2BatchDesc<C> D = uniform_load(d + p);
+    W2Args<float, C> a = D.a;
+    const bool src = a.src && 2 * pass < D.nsrc;  // values past the end are zero padding
+    a.src = src ? a.src + 2 * pass : nullptr;
+    a.src_k = src ? a.src_k : -1;
+    const int local = it - D.item0;
+    const int ntile = a.tiles_x * a.tiles_y;
+    WT::run(a, local % ntile, local / ntile, lds);
+    __syncthreads();  // LDS and the claim slots are reused by the next item
+  }
+}
+
 typedef void (*batch_fn)(const BatchDesc *, int, int, unsigned long long *, unsigned long long, int);
+typedef void (*w2_fn)(const W2BatchDesc<float> *, int, int, unsigned long long *, unsigned long long, int);
+
+// two-step kernels by register cap; the bf16 instantiation reads the same table
+// bytes (W2BatchDesc<bf16> and W2BatchDesc<float> differ in the roc2 pointer's type only)
+constexpr int kW2Minw[] = {1, 4};
+constexpr int kNumW2 = 2;
+w2_fn w2_kernel(bool bf16, int v) {
+  if (bf16)
+    return v ? reinterpret_cast<w2_fn>(&batch_w2_kernel<__hip_bfloat16, 4>)
+             : reinterpret_cast<w2_fn>(&batch_w2_kernel<__hip_bfloat16, 1>);
+  return v ? &batch_w2_kernel<float, 4> : &batch_w2_kernel<float, 1>;
+}
 
 struct BatchVariant {
   int bx, by, ry;
@@ -146,18 +210,28 @@ struct gr_batch {
   std::vector<gr_handle *> probs;  // in the caller's order
   std::vector<int> order;          // work order: largest grid first
   int variant = 0, zchunk = 0;
+  int tblock = 1;                  // time steps per launch: 1 (single steps) or 2 (two-step blocks)
   int nitems = 0, grid = 0;
   int occupancy[kNumVariants] = {};
   int ncu = 0;
   BatchDesc *ddesc = nullptr;  // [4][n]: step parity 0/1 of the problem fields, then of the tuning spares
   std::vector<BatchDesc> hdesc;
+  // two-step blocks: tables [0] first block of a call, [1] odd blocks, [2] even blocks
+  // after the first, [3] tuning (reads the fields, writes the spares); [4] the single
+  // step that ends an odd step count
+  W2BatchDesc<float> *dw2 = nullptr;
+  std::vector<W2BatchDesc<float>> hw2;
+  int w2_zchunk = 0;   // z planes per two-step item (0 = whole grid)
+  int w2_variant = 0;  // index into kW2Minw
+  int w2_items = 0, w2_grid = 0, w2_occupancy[kNumW2] = {};
+  int last_tblock = 1;  // steps per launch of the last call's blocks
   unsigned long long *ctr = nullptr;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   bool bf16 = false;
   // cached step-loop graph
   hipGraphExec_t gexec = nullptr;
-  int g_napply = -1, g_variant = -1, g_zchunk = -1, g_table = -1;
+  int g_napply = -1, g_variant = -1, g_zchunk = -1, g_table = -1, g_tblock = -1, g_w2_zchunk = -1, g_w2_variant = -1;
   double last_ms = 0;
   int last_launches = 0;
   long long last_updates = 0;
@@ -174,9 +248,10 @@ int items_of(const Problem &P, int zchunk, int tx_cells, int ty_cells, int *tx, 
 }
 
 // Fill the four descriptor tables: [0] / [1] advance the problems' fields from
-// their current parity (table 1 for odd steps of a call), [2] / [3] ping-pong
-// between the two spare fields (tuning runs; no source).
-int build_tables(gr_batch *b, bool spares) {
+// their parity (table 1 for odd steps of a call), [2] / [3] ping-pong between the
+// two spare fields (tuning runs; no source).  The fields are those the problems
+// hold after `npasses` two-step blocks (0: their current fields).
+int build_tables(gr_batch *b, bool spares, int npasses = 0) {
   const int n = (int)b->probs.size();
   const BatchVariant &V = kVariants[b->variant];
   b->hdesc.assign(4 * (size_t)n, BatchDesc{});
@@ -203,11 +278,17 @@ int build_tables(gr_batch *b, bool spares) {
     const float *src = (const float *)gr_internal_source(h, &nsrc, &sidx);
     const long long done = gr_internal_napplied(h);
     const bool has_src = src && sidx >= 0 && done < nsrc;
+    void *buf[4];  // written / read field at the problem's parity, then its spares
+    if (npasses > 0)
+      gr_internal_fused_buffers(h, npasses, buf);
+    else {
+      buf[0] = gr_internal_field(h, par);
+      buf[1] = gr_internal_field(h, 1 - par);
+    }
     for (int q = 0; q < 2; ++q) {
       BatchDesc E = D;
-      const int w = par ^ q;  // logical field written at this step parity
-      E.u = (float *)gr_internal_field(h, w);
-      E.v = (const float *)gr_internal_field(h, 1 - w);
+      E.u = (float *)buf[q];  // step parity q of the call writes the field read at parity 1 - q
+      E.v = (const float *)buf[1 - q];
       E.src = has_src ? src + done : nullptr;
       E.nsrc = has_src ? (int)(nsrc - done) : 0;
       E.src_k = has_src ? (int)(sidx / D.sz) : -1;
@@ -232,14 +313,91 @@ int build_tables(gr_batch *b, bool spares) {
   return GR_OK;
 }
 
-hipError_t enqueue_steps(gr_batch *b, int table, int napply, hipStream_t st) {
+// Fill the two-step tables: [0] the first block of a call, [1] its odd blocks,
+// [2] its even blocks after the first (the buffer rotation of the single-problem
+// path has period two from the second block on), [3] tuning blocks, which read the
+// problems' fields and write their spares (no source).
+int build_w2_tables(gr_batch *b) {
+  const int n = (int)b->probs.size();
+  b->hw2.assign(4 * (size_t)n, W2BatchDesc<float>{});
+  int item = 0;
+  for (int oi = 0; oi < n; ++oi) {
+    gr_handle *h = b->probs[b->order[oi]];
+    const Problem &P = gr_internal_problem(h);
+    W2BatchDesc<float> D{};
+    W2Args<float, float> &a = D.a;
+    a.roc2 = (const float *)P.roc2;  // a pointer either way: bf16 tables are read as W2BatchDesc<bf16>
+    for (int r = 0; r < 5; ++r) a.c[r] = (float)P.c[r];
+    a.csum = fast_csum(P.c);
+    a.sz = (long long)P.pitch * P.rows;
+    a.sy = P.pitch;
+    a.nx = P.nx;
+    a.ny = P.ny;
+    a.nz = P.nz;
+    const int cnt = items_of(P, b->w2_zchunk, W2Tile::TX, W2Tile::TY, &a.tiles_x, &a.tiles_y, &a.nzc, &a.zchunk);
+    D.item0 = item;
+    item += cnt;
+    int nsrc = 0;
+    long long sidx = -1;
+    const float *src = (const float *)gr_internal_source(h, &nsrc, &sidx);
+    const long long done = gr_internal_napplied(h);
+    const bool has_src = src && sidx >= 0 && done < nsrc;
+    for (int t = 0; t < 4; ++t) {
+      W2BatchDesc<float> E = D;
+      void *buf[4];
+      gr_internal_fused_buffers(h, t < 3 ? t : 0, buf);
+      E.a.P = (const float *)buf[0];  // u^(n-1)
+      E.a.V = (const float *)buf[1];  // u^n
+      E.a.W = (float *)buf[2];
+      E.a.Y = (float *)buf[3];
+      if (t == 3) {  // tuning: the fields are read, the spares written
+        void *sp0 = gr_internal_spare(h, 0), *sp1 = gr_internal_spare(h, 1);
+        if (!sp0 || !sp1) return set_error(GR_ENOMEM, "gr_batch: cannot allocate tuning spares");
+        E.a.P = (const float *)gr_internal_field(h, 0);
+        E.a.V = (const float *)gr_internal_field(h, 1);
+        E.a.W = (float *)sp0;
+        E.a.Y = (float *)sp1;
+      }
+      const bool s = has_src && t < 3;
+      E.a.src = s ? src + done : nullptr;
+      E.a.src_k = s ? (int)(sidx / a.sz) : -1;
+      E.a.src_off = s ? (int)(sidx % a.sz) : 0;
+      E.nsrc = s ? (int)(nsrc - done) : 0;
+      b->hw2[(size_t)t * n + oi] = E;
+    }
+  }
+  b->w2_items = item;
+  b->w2_grid = std::max(1, std::min(item, b->ncu * std::max(1, b->w2_occupancy[b->w2_variant])));
+  hipError_t e = hipMemcpyAsync(b->dw2, b->hw2.data(), sizeof(W2BatchDesc<float>) * b->hw2.size(),
+                                hipMemcpyHostToDevice, b->stream);
+  if (e != hipSuccess) return set_errorf(GR_EHIP, "gr_batch: descriptor copy: %s", hipGetErrorString(e));
+  return GR_OK;
+}
+
+// Enqueue one call: napply single steps from table `table` (0: the problems, 2:
+// spares), or with tblock == 2, napply / 2 two-step blocks (table 3 of the
+// two-step tables when tuning) and, for an odd count, one single step from table 0
+// (built for the fields after the blocks).  The claim counter's base advances by
+// items + grid per launch.
+hipError_t enqueue_steps(gr_batch *b, int table, int napply, int tblock, bool tuning, hipStream_t st) {
   const int n = (int)b->probs.size();
   const BatchVariant &V = kVariants[b->variant];
   const batch_fn f = b->bf16 ? V.bf16 : V.f32;
   hipLaunchKernelGGL(batch_reset_kernel, dim3(1), dim3(64), 0, st, b->ctr);
   unsigned long long base = 0;
-  for (int s = 0; s < napply; ++s) {
-    const BatchDesc *tab = b->ddesc + (size_t)(table + (s & 1)) * n;
+  int s = 0;
+  if (tblock == 2) {
+    for (int pass = 0; 2 * pass + 1 < napply; ++pass, s += 2) {
+      const int t = tuning ? 3 : pass == 0 ? 0 : (pass & 1) ? 1 : 2;
+      const W2BatchDesc<float> *tab = b->dw2 + (size_t)t * n;
+      hipLaunchKernelGGL(w2_kernel(b->bf16, b->w2_variant), dim3(b->w2_grid), dim3(kW2Lanes), 0, st, tab, n,
+                         b->w2_items, b->ctr, base, pass);
+      base += (unsigned long long)(b->w2_items + b->w2_grid);
+    }
+    table = 0;
+  }
+  for (int q = 0; s < napply; ++s, ++q) {
+    const BatchDesc *tab = b->ddesc + (size_t)(table + (q & 1)) * n;
     hipLaunchKernelGGL(f, dim3(b->grid), dim3(64 * V.bx, V.by), 0, st, tab, n, b->nitems, b->ctr, base, s);
     base += (unsigned long long)(b->nitems + b->grid);
   }
@@ -249,20 +407,22 @@ hipError_t enqueue_steps(gr_batch *b, int table, int napply, hipStream_t st) {
 void drop_graph(gr_batch *b) {
   if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
   b->gexec = nullptr;
-  b->g_napply = b->g_variant = b->g_zchunk = b->g_table = -1;
+  b->g_napply = b->g_variant = b->g_zchunk = b->g_table = b->g_tblock = b->g_w2_zchunk = b->g_w2_variant = -1;
 }
 
-// Run napply steps from descriptor table `table` (0: the problems, 2: spares).
-int run_steps(gr_batch *b, int table, int napply, bool graph) {
+// Run one call (enqueue_steps' arguments), timed, optionally replayed from a graph.
+int run_steps(gr_batch *b, int table, int napply, bool graph, int tblock = 1, bool tuning = false) {
   hipError_t e = hipSuccess;
+  const int gtable = tuning ? 3 : table;
   if (graph && napply > 0) {
     if (!b->gexec || b->g_napply != napply || b->g_variant != b->variant || b->g_zchunk != b->zchunk ||
-        b->g_table != table) {
+        b->g_table != gtable || b->g_tblock != tblock || b->g_w2_zchunk != b->w2_zchunk ||
+        b->g_w2_variant != b->w2_variant) {
       drop_graph(b);
       hipGraph_t g = nullptr;
       e = hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal);
       if (e == hipSuccess) {
-        const hipError_t e2 = enqueue_steps(b, table, napply, b->stream);
+        const hipError_t e2 = enqueue_steps(b, table, napply, tblock, tuning, b->stream);
         e = hipStreamEndCapture(b->stream, &g);
         if (e == hipSuccess) e = e2;
       }
@@ -275,13 +435,16 @@ int run_steps(gr_batch *b, int table, int napply, bool graph) {
       b->g_napply = napply;
       b->g_variant = b->variant;
       b->g_zchunk = b->zchunk;
-      b->g_table = table;
+      b->g_table = gtable;
+      b->g_tblock = tblock;
+      b->g_w2_zchunk = b->w2_zchunk;
+      b->g_w2_variant = b->w2_variant;
     }
     e = hipEventRecord(b->ev0, b->stream);
     if (e == hipSuccess) e = hipGraphLaunch(b->gexec, b->stream);
   } else {
     e = hipEventRecord(b->ev0, b->stream);
-    if (e == hipSuccess) e = enqueue_steps(b, table, napply, b->stream);
+    if (e == hipSuccess) e = enqueue_steps(b, table, napply, tblock, tuning, b->stream);
   }
   if (e == hipSuccess) e = hipEventRecord(b->ev1, b->stream);
   if (e == hipSuccess) e = hipEventSynchronize(b->ev1);
@@ -289,13 +452,26 @@ int run_steps(gr_batch *b, int table, int napply, bool graph) {
   float ms = 0.f;
   (void)hipEventElapsedTime(&ms, b->ev0, b->ev1);
   b->last_ms = ms;
-  b->last_launches = napply;
+  b->last_tblock = tblock;
+  b->last_launches = tblock == 2 ? napply / 2 + (napply & 1) : napply;
   long long up = 0;
   for (gr_handle *h : b->probs) {
     const Problem &P = gr_internal_problem(h);
     up += (long long)P.nx * P.ny * P.nz;
   }
   b->last_updates = up * napply;
+  return GR_OK;
+}
+
+// Whether every problem may run two-step blocks now (fields with equal halos;
+// spares allocated and holding that halo).
+int fused_ready(gr_batch *b, bool *ok) {
+  *ok = b->w2_occupancy[b->w2_variant] >= 1;
+  for (gr_handle *h : b->probs) {
+    if (!*ok) break;
+    const int rc = gr_internal_fused_prepare(h, ok);
+    if (rc) return rc;
+  }
   return GR_OK;
 }
 
@@ -315,6 +491,7 @@ extern "C" int gr_batch_destroy(gr_batch *b) {
   drop_graph(b);
   for (gr_handle *h : b->probs) gr_destroy(h);
   if (b->ddesc) (void)hipFree(b->ddesc);
+  if (b->dw2) (void)hipFree(b->dw2);
   if (b->ctr) (void)hipFree(b->ctr);
   if (b->ev0) (void)hipEventDestroy(b->ev0);
   if (b->ev1) (void)hipEventDestroy(b->ev1);
@@ -348,6 +525,7 @@ extern "C" int gr_batch_create(const gr_desc *descs, int n, gr_batch **out) {
   if (rc == GR_OK) {
     int dev = 0;
     e = hipMalloc(&b->ddesc, sizeof(BatchDesc) * 4 * (size_t)n);
+    if (e == hipSuccess) e = hipMalloc(&b->dw2, sizeof(W2BatchDesc<float>) * 4 * (size_t)n);
     if (e == hipSuccess) e = hipMalloc(&b->ctr, sizeof(unsigned long long));
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreate(&b->ev0);
@@ -359,6 +537,11 @@ extern "C" int gr_batch_create(const gr_desc *descs, int n, gr_batch **out) {
       e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
           &b->occupancy[v], reinterpret_cast<const void *>(b->bf16 ? V.bf16 : V.f32), 64 * V.bx * V.by, 0);
     }
+    for (int v = 0; v < kNumW2 && e == hipSuccess; ++v)
+      e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b->w2_occupancy[v],
+                                                       reinterpret_cast<const void *>(w2_kernel(b->bf16, v)),
+                                                       kW2Lanes, 0);
+    if (e == hipSuccess && b->w2_occupancy[0] < 1) b->w2_variant = 1;
     if (e != hipSuccess) rc = set_errorf(GR_EHIP, "gr_batch_create: %s", hipGetErrorString(e));
   }
   if (rc != GR_OK) {
@@ -387,7 +570,7 @@ extern "C" int gr_batch_set_config(gr_batch *b, int variant, int zchunk) {
   if (variant >= kNumVariants)
     return set_errorf(GR_EINVAL, "gr_batch_set_config: variant %d (0..%d)", variant, kNumVariants - 1);
   if (variant >= 0) b->variant = variant;
-  if (zchunk >= 0) b->zchunk = zchunk;
+  if (zchunk >= 0) b->zchunk = b->w2_zchunk = zchunk;
   return GR_OK;
 }
 
@@ -395,20 +578,46 @@ extern "C" int gr_batch_step(gr_batch *b, int napply, int use_graph) {
   if (!b) return set_error(GR_EINVAL, "gr_batch_step: null batch");
   if (napply < 0) return set_error(GR_EINVAL, "gr_batch_step: negative step count");
   int rc = sync_problems(b);
-  if (rc == GR_OK) rc = build_tables(b, false);
-  if (rc == GR_OK) rc = run_steps(b, 0, napply, use_graph != 0);
+  if (rc) return rc;
+  bool fuse = false;
+  if (b->tblock == 2 && napply >= 2 && (rc = fused_ready(b, &fuse))) return rc;
+  const int npasses = fuse ? napply / 2 : 0;
+  if (fuse) rc = build_w2_tables(b);
+  if (rc == GR_OK) rc = build_tables(b, false, npasses);
+  if (rc == GR_OK) rc = run_steps(b, 0, napply, use_graph != 0, fuse ? 2 : 1);
   if (rc != GR_OK) return rc;
-  for (gr_handle *h : b->probs) gr_internal_advance(h, napply, b->last_ms);
+  for (gr_handle *h : b->probs) {
+    if (fuse)
+      gr_internal_fused_advance(h, npasses, napply, b->last_ms);
+    else
+      gr_internal_advance(h, napply, b->last_ms);
+  }
+  return GR_OK;
+}
+
+extern "C" int gr_batch_set_time_block(gr_batch *b, int tblock, int block_variant) {
+  if (!b) return set_error(GR_EINVAL, "gr_batch_set_time_block: null batch");
+  if (tblock != 1 && tblock != 2)
+    return set_errorf(GR_EUNSUPPORTED, "gr_batch_set_time_block: %d steps per launch (1 or 2)", tblock);
+  if (block_variant >= kNumW2)
+    return set_errorf(GR_EINVAL, "gr_batch_set_time_block: block variant %d (0..%d)", block_variant, kNumW2 - 1);
+  const int wv = block_variant >= 0 ? block_variant : b->w2_variant;
+  if (tblock == 2 && b->w2_occupancy[wv] < 1)
+    return set_error(GR_EUNSUPPORTED, "gr_batch_set_time_block: the two-step kernel does not fit a CU");
+  b->tblock = tblock;
+  b->w2_variant = wv;
   return GR_OK;
 }
 
 extern "C" int gr_batch_autotune(gr_batch *b, int steps, int use_graph, int *variant, int *zchunk) {
   if (!b) return set_error(GR_EINVAL, "gr_batch_autotune: null batch");
   if (steps < 1) steps = 4;
+  for (gr_handle *h : b->probs)  // allocate the spares up front: their set-up runs on the problems' streams
+    if (!gr_internal_spare(h, 0)) return set_error(GR_ENOMEM, "gr_batch: cannot allocate tuning spares");
   int rc = sync_problems(b);
   if (rc) return rc;
   const int zcands[] = {0, 256, 128, 64};
-  double best = 1e30;
+  double best = 1e30;  // ms per step
   int bv = b->variant, bz = b->zchunk;
   for (int v = 0; v < kNumVariants; ++v) {
     if (b->occupancy[v] < 1) continue;
@@ -418,18 +627,42 @@ extern "C" int gr_batch_autotune(gr_batch *b, int steps, int use_graph, int *var
       if ((rc = build_tables(b, true))) return rc;
       if ((rc = run_steps(b, 2, 1, false))) return rc;  // warm-up
       if ((rc = run_steps(b, 2, steps, use_graph != 0))) return rc;
-      if (b->last_ms < best) {
-        best = b->last_ms;
+      if (b->last_ms / steps < best) {
+        best = b->last_ms / steps;
         bv = v;
         bz = zc;
       }
     }
   }
-  drop_graph(b);
   b->variant = bv;
   b->zchunk = bz;
+  // two-step blocks, timed per step against the best single step
+  int btb = 1, bwz = b->w2_zchunk, bwv = b->w2_variant;
+  bool fuse = false;
+  if ((rc = fused_ready(b, &fuse))) return rc;
+  const int passes = std::max(1, steps / 2);
+  for (int wv = 0; wv < kNumW2 && fuse; ++wv) {
+    if (b->w2_occupancy[wv] < 1) continue;
+    for (int zc : zcands) {
+      b->w2_variant = wv;
+      b->w2_zchunk = zc;
+      if ((rc = build_w2_tables(b))) return rc;
+      if ((rc = run_steps(b, 0, 2, false, 2, true))) return rc;  // warm-up
+      if ((rc = run_steps(b, 0, 2 * passes, use_graph != 0, 2, true))) return rc;
+      if (b->last_ms / (2 * passes) < best) {
+        best = b->last_ms / (2 * passes);
+        btb = 2;
+        bwz = zc;
+        bwv = wv;
+      }
+    }
+  }
+  drop_graph(b);
+  b->tblock = btb;
+  b->w2_zchunk = bwz;
+  b->w2_variant = bwv;
   if (variant) *variant = bv;
-  if (zchunk) *zchunk = bz;
+  if (zchunk) *zchunk = btb == 2 ? bwz : bz;
   return GR_OK;
 }
 
@@ -447,6 +680,15 @@ extern "C" int gr_batch_get_stats(gr_batch *b, gr_batch_stats *out) {
   s.tile_x = 64 * kVariants[b->variant].bx;
   s.tile_y = kVariants[b->variant].by * kVariants[b->variant].ry;
   s.nvariants = kNumVariants;
+  s.time_block = b->tblock;
+  s.block_variant = b->w2_variant;
+  if (b->last_tblock == 2) {  // the last call ran two-step blocks: their items and tiles
+    s.items = b->w2_items;
+    s.grid = b->w2_grid;
+    s.zchunk = b->w2_zchunk;
+    s.tile_x = W2Tile::TX;
+    s.tile_y = W2Tile::TY;
+  }
   *out = s;
   return GR_OK;
 }

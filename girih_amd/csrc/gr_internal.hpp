@@ -64,3 +64,11 @@ const void *gr_internal_source(const gr_handle *h, int *n, long long *idx);
 hipStream_t gr_internal_stream(const gr_handle *h);
 const gr::LaunchCfg &gr_internal_launch_cfg(const gr_handle *h);
 void gr_internal_advance(gr_handle *h, int napply, double ms);
+// Two-step kernel-0 passes for the batch: prepare (spares allocated and given the
+// fields' halo; *ok false when the fields' halos differ and passes may not run), the
+// buffers {read as u^(n-1), read as u^n, written as u^(n+1), written as u^(n+2)} of
+// pass number npasses of a call, and the state after npasses passes plus
+// napply - 2*npasses single steps.
+int gr_internal_fused_prepare(gr_handle *h, bool *ok);
+void gr_internal_fused_buffers(const gr_handle *h, int npasses, void *buf[4]);
+void gr_internal_fused_advance(gr_handle *h, int npasses, int napply, double ms);

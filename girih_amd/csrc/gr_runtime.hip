@@ -488,7 +488,8 @@ static int fused_depth_ok(gr_handle *h, int tl, bool *ok) {
   bool have = false;
   for (int i = 0; i < nd; ++i) have |= depths[i] == tl;
   if (!have) return GR_OK;
-  if ((h->P.flags & GR_COEF_BF16) != 0 && h->P.kernel != 5) return GR_OK;  // bf16 fused: kernel 5 only
+  // bf16-stored fields fuse for kernel 5 (coefficient arrays) and kernel 0 (roc2)
+  if ((h->P.flags & GR_COEF_BF16) != 0 && h->P.kernel != 5 && h->P.kernel != 0) return GR_OK;
   int rc = check_halo_equal(h);
   if (rc) return rc;
   *ok = h->halo_equal == 1;
@@ -1265,6 +1266,57 @@ const void *gr_internal_source(const gr_handle *h, int *n, long long *idx) {
 }
 hipStream_t gr_internal_stream(const gr_handle *h) { return h->stream; }
 const gr::LaunchCfg &gr_internal_launch_cfg(const gr_handle *h) { return h->lc; }
+int gr_internal_fused_prepare(gr_handle *h, bool *ok) {
+  *ok = false;
+  bool depth_ok = false;
+  int rc = fused_depth_ok(h, 2, &depth_ok);
+  if (rc || !depth_ok || h->P.kernel != 0) return rc;
+  if ((rc = ensure_spares(h))) return rc;
+  if (!h->spares_synced) {  // the spares need the fields' halo (never written by a pass)
+    int sp[2];
+    spare_ids(h, sp);
+    const size_t fbytes = (size_t)h->P.field_elems * elem_size(h->P.dtype);
+    HIP_TRY(hipMemcpyAsync(h->fbuf[sp[0]], h->fbuf[h->lu[1]], fbytes, hipMemcpyDeviceToDevice, h->stream));
+    HIP_TRY(hipMemcpyAsync(h->fbuf[sp[1]], h->fbuf[h->lu[1]], fbytes, hipMemcpyDeviceToDevice, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    h->spares_synced = true;
+  }
+  *ok = true;
+  return GR_OK;
+}
+void gr_internal_fused_buffers(const gr_handle *h, int npasses, void *buf[4]) {
+  // the mapping (written-next, read-next) after npasses two-step passes, and its spares;
+  // a pass writes (u^(n+1), u^(n+2)) into the spares, which then become the fields
+  int lu[2] = {h->lu[0], h->lu[1]};
+  const int w = h->parity, r = 1 - w;
+  for (int p = 0; p <= npasses; ++p) {
+    int sp[2], n = 0;
+    for (int i = 0; i < 4 && n < 2; ++i)
+      if (i != lu[0] && i != lu[1]) sp[n++] = i;
+    if (p == npasses) {
+      buf[0] = field_base(h, h->fbuf[lu[w]]);
+      buf[1] = field_base(h, h->fbuf[lu[r]]);
+      buf[2] = field_base(h, h->fbuf[sp[0]]);
+      buf[3] = field_base(h, h->fbuf[sp[1]]);
+      return;
+    }
+    lu[w] = sp[0];
+    lu[r] = sp[1];
+  }
+}
+void gr_internal_fused_advance(gr_handle *h, int npasses, int napply, double ms) {
+  for (int p = 0; p < npasses; ++p) {  // the rotation of gr_step's kernel-0 fused path
+    int sp[2];
+    spare_ids(h, sp);
+    h->lu[h->parity] = sp[0];
+    h->lu[1 - h->parity] = sp[1];
+  }
+  h->parity ^= (napply - 2 * npasses) & 1;
+  h->napplied += napply;
+  h->stats.last_ms = ms;
+  h->stats.last_updates = (int64_t)napply * h->P.nx * h->P.ny * h->P.nz;
+  h->stats.last_launches = npasses + (napply - 2 * npasses);
+}
 void gr_internal_advance(gr_handle *h, int napply, double ms) {
   h->parity ^= napply & 1;
   h->napplied += napply;

@@ -62,7 +62,8 @@ class BatchStats(ctypes.Structure):
     _fields_ = [("last_ms", ctypes.c_double), ("last_updates", ctypes.c_int64), ("last_launches", ctypes.c_int32),
                 ("nproblems", ctypes.c_int32), ("items", ctypes.c_int32), ("grid", ctypes.c_int32),
                 ("variant", ctypes.c_int32), ("zchunk", ctypes.c_int32), ("tile_x", ctypes.c_int32),
-                ("tile_y", ctypes.c_int32), ("nvariants", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+                ("tile_y", ctypes.c_int32), ("nvariants", ctypes.c_int32), ("time_block", ctypes.c_int32),
+                ("block_variant", ctypes.c_int32), ("reserved", ctypes.c_int32)]
 
 
 EXPORTS = ["gr_create", "gr_destroy", "gr_last_error", "gr_upload", "gr_download", "gr_step", "gr_run_ts",
@@ -71,7 +72,7 @@ EXPORTS = ["gr_create", "gr_destroy", "gr_last_error", "gr_upload", "gr_download
            "gr_set_graph", "gr_autotune",
            "gr_init_roc2_model",
            "gr_download_roc2", "gr_batch_create", "gr_batch_destroy", "gr_batch_problem", "gr_batch_set_config",
-           "gr_batch_step", "gr_batch_autotune", "gr_batch_get_stats", "gr_shot_create", "gr_shot_destroy",
+           "gr_batch_step", "gr_batch_set_time_block", "gr_batch_autotune", "gr_batch_get_stats", "gr_shot_create", "gr_shot_destroy",
            "gr_set_roc2_from_slowness2", "gr_shot_forward", "gr_shot_adjoint"]
 
 _lib = None
@@ -122,6 +123,7 @@ def lib():
             "gr_batch_problem": ([P, I, ctypes.POINTER(P)], I),
             "gr_batch_set_config": ([P, I, I], I),
             "gr_batch_step": ([P, I, I], I),
+            "gr_batch_set_time_block": ([P, I, I], I),
             "gr_batch_autotune": ([P, I, I, IP, IP], I),
             "gr_batch_get_stats": ([P, ctypes.POINTER(BatchStats)], I),
             "gr_shot_create": ([P, I, I, I, ctypes.c_double, ctypes.POINTER(P)], I),
@@ -325,7 +327,7 @@ class Stepper:
 
 class Batch:
     """Independent kernel-0 problems of uneven size advanced together, one
-    persistent work-queue launch per time step (gr_batch_*)."""
+    persistent work-queue launch per time step or per two-step block (gr_batch_*)."""
 
     def __init__(self, sizes, flags=GR_FAST | GR_COEF_BF16, coef=None):
         self.sizes = [tuple(int(x) for x in s) for s in sizes]
@@ -371,6 +373,11 @@ class Batch:
 
     def set_config(self, variant=-1, zchunk=-1):
         check(lib().gr_batch_set_config(self._b, int(variant), int(zchunk)))
+
+    def set_time_block(self, tblock, block_variant=-1):
+        """1: one launch per step; 2: two-step blocks, block_variant 0 (registers
+        uncapped) or 1 (capped at 128) (gr_batch_set_time_block)."""
+        check(lib().gr_batch_set_time_block(self._b, int(tblock), int(block_variant)))
 
     def autotune(self, steps=4, use_graph=False):
         v, z = ctypes.c_int(0), ctypes.c_int(0)

@@ -15,8 +15,10 @@ one untimed pass (--warmup 0 drops it).
 
 --batch runs the 16-problem wave batch instead (kernel 0, FP32 GR_FAST, bf16 roc2
 from 8-layer velocity models, dims uniform in 192..512 step 16 from seed 2024, a
-Ricker source at each centre), with --config VARIANT,ZCHUNK of gr_batch_set_config
-or the batch autotuner, one persistent launch per step replayed from a HIP graph.
+Ricker source at each centre), with --config VARIANT,ZCHUNK[,TBLOCK[,BLOCK_VARIANT]]
+of gr_batch_set_config and gr_batch_set_time_block, or the batch autotuner; one
+persistent launch per step (TBLOCK 1) or per two-step block (2), replayed from a HIP
+graph.
 """
 import argparse
 import json
@@ -39,15 +41,17 @@ def batch_main(a):
             nx, ny, nz = sizes[i]
             p.set_source(4 + nx // 2, 4 + ny // 2, 4 + nz // 2, src)
         if a.config:
-            var, zc = (int(x) for x in a.config.split(","))
-            b.set_config(var, zc)
+            cfg = [int(x) for x in a.config.split(",")] + [1, -1]
+            b.set_config(cfg[0], cfg[1])
+            b.set_time_block(cfg[2], cfg[3])
         else:
             b.autotune(4, use_graph=True)
         b.step(2, use_graph=True)
         b.step(a.steps, use_graph=True)
         s = b.stats()
-        print(json.dumps({"batch": len(sizes), "cells": cells, "config": "%d,%d" % (s["variant"], s["zchunk"]),
-                          "tile": [s["tile_x"], s["tile_y"]], "steps": a.steps,
+        print(json.dumps({"batch": len(sizes), "cells": cells,
+                          "config": "%d,%d,%d,%d" % (s["variant"], s["zchunk"], s["time_block"], s["block_variant"]),
+                          "tile": [s["tile_x"], s["tile_y"]], "steps": a.steps, "launches": s["last_launches"],
                           "glups": cells * a.steps / s["last_ms"] / 1e6}))
 
 

@@ -152,7 +152,7 @@ int gr_init_coef_uniform(gr_handle *h, uint64_t seed, double lo0, double hi0, do
  * gr_get_stats for the resulting tile), z planes streamed per workgroup (0 = all),
  * fused time steps per HBM pass (1 = single sweep; kernel 0: 2; kernel 1: 2..6;
  * kernel 5: 2, 4, 6; other depths return GR_EUNSUPPORTED).  Fused passes run only
- * while U1 and U2 hold the same halo, and with GR_COEF_BF16 for kernel 5 only;
+ * while U1 and U2 hold the same halo (with GR_COEF_BF16: kernels 0 and 5);
  * otherwise the stepper uses single sweeps.  Values < 0 leave a setting unchanged.  Results
  * never depend on the configuration. */
 int gr_set_config(gr_handle *h, int tile_variant, int zchunk, int t_block);
@@ -231,21 +231,37 @@ typedef struct gr_batch_stats {
   int32_t variant, zchunk;  /* tile variant and z planes per item (0 = whole grid) */
   int32_t tile_x, tile_y;
   int32_t nvariants;        /* tile variants available */
+  int32_t time_block;       /* configured steps per launch (1 or 2); items, grid, zchunk and
+                               tile_x/y describe the two-step blocks when the last call ran them */
+  int32_t block_variant;    /* two-step kernel (gr_batch_set_time_block) */
   int32_t reserved;
 } gr_batch_stats;
 int gr_batch_create(const gr_desc *descs, int n, gr_batch **out);
 int gr_batch_destroy(gr_batch *b);
 /* Handle of problem i (owned by the batch: do not gr_destroy it). */
 int gr_batch_problem(gr_batch *b, int i, gr_handle **h);
-/* Tile variant (0..nvariants-1) and z planes per work item (0 = whole grid);
- * values < 0 leave a setting unchanged.  Results never depend on them. */
+/* Tile variant of single steps (0..nvariants-1) and z planes per work item
+ * (0 = whole grid; single steps and two-step blocks); values < 0 leave a setting
+ * unchanged.  Results never depend on them. */
 int gr_batch_set_config(gr_batch *b, int variant, int zchunk);
 /* Advance every problem by napply applications (girih's U1/U2 pairing per
  * problem).  use_graph != 0 replays the step loop from a HIP graph (captured on
  * first use, kept while napply and the configuration stay the same). */
 int gr_batch_step(gr_batch *b, int napply, int use_graph);
-/* Time every tile variant x z chunk for `steps` steps on scratch fields and keep
- * the fastest; the problems' fields and parities are untouched. */
+/* Time steps per launch (no counterpart in girih, whose diamonds block time per
+ * grid, src/kernels/diamond_ts.c): 1 = single steps; 2 = two-step blocks, where a
+ * work item is (problem, 64 x 16 tile, z chunk) advanced two steps on chip, reading
+ * (u^(n-1), u^n) and writing (u^(n+1), u^(n+2)) once (8 B per update in FP32
+ * against 12).  A call with an odd step count ends with one single step.  Blocks
+ * run only while every problem's U1 and U2 hold the same halo; otherwise a call
+ * runs single steps.  Other tblock values return GR_EUNSUPPORTED.  block_variant
+ * selects the two-step kernel: 0 keeps every value in registers (one workgroup
+ * per CU), 1 caps them at 128 (two per CU, a few spills); < 0 leaves it unchanged.
+ * Results never depend on either. */
+int gr_batch_set_time_block(gr_batch *b, int tblock, int block_variant);
+/* Time every tile variant x z chunk for `steps` steps, then two-step blocks per z
+ * chunk, on scratch fields, and keep the fastest per step (*zchunk reports the
+ * chosen kind's chunk); the problems' fields and parities are untouched. */
 int gr_batch_autotune(gr_batch *b, int steps, int use_graph, int *variant, int *zchunk);
 int gr_batch_get_stats(gr_batch *b, gr_batch_stats *out);
 

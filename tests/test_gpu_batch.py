@@ -3,6 +3,11 @@
 * every problem of a batch ends bit-identical to the same problem stepped alone
   with gr_step (same GR_FAST arithmetic, bf16 or FP32 roc2), whatever the tile
   variant, z chunk, step parity or source;
+* two-step blocks (gr_batch_set_time_block) give the same bits as single steps,
+  for both two-step kernels, any z chunk, odd and even step counts, sources that
+  end inside a block, problems at mixed parities and problems whose fields were
+  rotated by their own fused passes; a problem whose U1 and U2 halos differ makes
+  the call fall back to single steps;
 * graph replays of the step loop equal eager launches on every replay;
 * against the FP64 CPU oracle, with the roc2 the batch actually holds, the FP32
   batch stays within a relative L2 error of 1e-5;
@@ -92,13 +97,102 @@ def test_batch_mixed_parities_and_graph_replays():
             np.testing.assert_array_equal(E[1], U2)
 
 
+@pytest.mark.parametrize("bf16", [True, False])
+@pytest.mark.parametrize("block_variant,zchunk", [(0, 0), (1, 0), (0, 5), (1, 8), (0, 1)])
+def test_batch_two_step_blocks_equal_single_problem_steps(bf16, block_variant, zchunk):
+    flags = gpu.GR_FAST | (gpu.GR_COEF_BF16 if bf16 else 0)
+    src = np.linspace(0.5, -0.2, 8)  # the third call's first block holds the last value and a zero
+    print("batch two-step", bf16, block_variant, zchunk, flush=True)
+    with gpu.Batch(SIZES, flags=flags, coef=physics.fd8_coef()) as b:
+        _init(b, 13, src)
+        b.set_time_block(2, block_variant)
+        b.set_config(-1, zchunk)
+        for n, launches in ((3, 2), (4, 2), (5, 3)):
+            b.step(n)
+            st = b.stats()
+            assert st["last_launches"] == launches, (n, st)
+            assert st["time_block"] == 2 and st["block_variant"] == block_variant
+            assert st["zchunk"] == zchunk and (st["tile_x"], st["tile_y"]) == (64, 16)
+        got = [p.download() for p in b.problems]
+    for i, size in enumerate(SIZES):
+        U1, U2 = _alone(size, flags, 13, i, 12, src)
+        np.testing.assert_array_equal(got[i][0], U1, err_msg="problem %d U1" % i)
+        np.testing.assert_array_equal(got[i][1], U2, err_msg="problem %d U2" % i)
+
+
+@pytest.mark.parametrize("bf16", [True, False])
+def test_batch_two_step_mixed_states_and_graph_replays(bf16):
+    """Problems one and three steps ahead, one whose own fused passes rotated its
+    buffers, odd step counts (each call ends with a single step and flips the
+    parity), replayed from a graph: every replay equals eager launches, and the
+    end state equals each problem stepped alone."""
+    flags = gpu.GR_FAST | (gpu.GR_COEF_BF16 if bf16 else 0)
+    pre = {1: 1, 2: 2, 3: 3}
+    with gpu.Batch(SIZES, flags=flags, coef=physics.fd8_coef()) as eager, \
+            gpu.Batch(SIZES, flags=flags, coef=physics.fd8_coef()) as graph:
+        for b in (eager, graph):
+            _init(b, 9)
+            b.problems[1].step(1)
+            b.problems[2].set_config(t_block=2)  # two-step pass on its own handle: spares become the fields
+            b.problems[2].step(2)
+            assert b.problems[2].stats()["last_launches"] == 1
+            b.problems[3].step(3)
+            b.set_time_block(2, 0)
+        for rep in range(3):
+            print("two-step replay", rep, flush=True)
+            eager.step(7)
+            graph.step(7, use_graph=True)
+            assert graph.stats()["last_launches"] == 4
+            for i in range(len(SIZES)):
+                E = eager.problems[i].download()
+                G = graph.problems[i].download()
+                np.testing.assert_array_equal(E[0], G[0])
+                np.testing.assert_array_equal(E[1], G[1])
+        for i in range(len(SIZES)):
+            U1, U2 = _alone(SIZES[i], flags, 9, i, 21, pre=pre.get(i, 0))
+            E = eager.problems[i].download()
+            np.testing.assert_array_equal(E[0], U1, err_msg="problem %d U1" % i)
+            np.testing.assert_array_equal(E[1], U2, err_msg="problem %d U2" % i)
+
+
+def test_batch_two_step_falls_back_on_unequal_halos():
+    flags = gpu.GR_FAST
+    outs = []
+    for tblock in (2, 1):
+        with gpu.Batch(SIZES, flags=flags, coef=physics.fd8_coef()) as b:
+            _init(b, 17)
+            U1, U2 = b.problems[2].download()
+            U1[1, 2, 3] = 0.25  # a halo cell of U1 only
+            b.problems[2].upload(U1, U2)
+            b.set_time_block(tblock)
+            b.step(4)
+            assert b.stats()["last_launches"] == 4
+            outs.append([p.download() for p in b.problems])
+    for (a1, a2), (b1, b2) in zip(*outs):
+        np.testing.assert_array_equal(a1, b1)
+        np.testing.assert_array_equal(a2, b2)
+
+
+def test_batch_time_block_errors():
+    with gpu.Batch(SIZES[:2], flags=gpu.GR_FAST, coef=physics.fd8_coef()) as b:
+        for bad in (0, 3, 4):
+            with pytest.raises(gpu.GirihError):
+                b.set_time_block(bad)
+        with pytest.raises(gpu.GirihError):
+            b.set_time_block(2, 2)
+        b.set_time_block(1)
+        assert b.stats()["time_block"] == 1
+
+
 def test_batch_autotune_keeps_fields():
     flags = gpu.GR_FAST | gpu.GR_COEF_BF16
     with gpu.Batch(SIZES[:3], flags=flags, coef=physics.fd8_coef()) as b:
         _init(b, 21)
         before = [p.download() for p in b.problems]
         v, z = b.autotune(2)
-        assert 0 <= v < b.stats()["nvariants"]
+        st = b.stats()
+        assert 0 <= v < st["nvariants"] and st["time_block"] in (1, 2)
+        print("autotune", v, z, st, flush=True)
         after = [p.download() for p in b.problems]
         for x, y in zip(before, after):
             np.testing.assert_array_equal(x[0], y[0])

@@ -82,22 +82,53 @@ def test_fused_bf16_coefficients_bitexact_vs_oracle(depth, zchunk):
         np.testing.assert_array_equal(U2[:, :, :nnx], v)
 
 
-def test_bf16_roc2_wave_keeps_single_sweeps():
-    """The two-step wave pass reads roc2 in the field type only: with a bf16 roc2
-    field, kernel 0 stays on single sweeps (and on the oracle's results)."""
+@pytest.mark.parametrize("zchunk", [0, 5])
+def test_bf16_roc2_wave_two_step_passes(zchunk):
+    """The two-step wave pass reads a bf16 roc2 field widened exactly to FP32: two
+    passes for four steps, bit-identical to the oracle on the bf16 values."""
     kernel, (nx, ny, nz) = 0, SIZES[4][0]
     u, v, roc2, coef = _random_problem(kernel, nx, ny, nz, np.float32, 733)
     roc2 = _bf16_round(roc2)
     with gpu.Stepper(kernel, nx, ny, nz, dtype=gpu.GR_F32, flags=gpu.GR_COEF_BF16, coef=_consts(coef, kernel)) as st:
         st.upload(_pad(u), _pad(v), _pad(roc2), None)
-        st.set_config(t_block=2)
+        st.set_config(zchunk=zchunk, t_block=2)
         st.step(4)
-        assert st.stats()["last_launches"] == 4
+        assert st.stats()["last_launches"] == 2
         U1, U2 = st.download(_pad(u).shape)
     orc.run(kernel, coef, u, v, roc2, nt=4)
     nnx = u.shape[2]
     np.testing.assert_array_equal(U1[:, :, :nnx], u)
     np.testing.assert_array_equal(U2[:, :, :nnx], v)
+
+
+def test_bf16_roc2_fast_two_step_bitexact_vs_oracle():
+    """GR_FAST | GR_COEF_BF16 (the batch's arithmetic and storage) in two-step
+    passes, with a source: bit-identical to the oracle's GR_FAST restatement run on
+    the bf16 roc2 values."""
+    nx, ny, nz = 150, 37, 21
+    rng = np.random.default_rng(91)
+    shp = orc.shape_of(0, nx, ny, nz)
+    u = np.zeros(shp, np.float32)
+    v = np.zeros(shp, np.float32)
+    u[4:-4, 4:-4, 4:-4] = rng.uniform(-1, 1, (nz, ny, nx))
+    v[4:-4, 4:-4, 4:-4] = rng.uniform(-1, 1, (nz, ny, nx))
+    roc2 = np.zeros(shp, np.float32)
+    roc2[4:-4, 4:-4, 4:-4] = _bf16_round(rng.uniform(0.02, 0.16, (nz, ny, nx)).astype(np.float32))
+    w = [-205.0 / 72, 8.0 / 5, -1.0 / 5, 8.0 / 315, -1.0 / 560]
+    coef = [3 * w[0]] + w[1:5]
+    src = np.linspace(1.0, -1.0, 9)
+    pt = (4 + 64, 4 + 16, 4 + 9)
+    with gpu.Stepper(0, nx, ny, nz, dtype=gpu.GR_F32, flags=gpu.GR_FAST | gpu.GR_COEF_BF16, coef=coef) as st:
+        st.upload(_pad(u), _pad(v), _pad(roc2), None)
+        st.set_source(*pt, src)
+        st.set_config(zchunk=6, t_block=2)
+        st.step(10)
+        assert st.stats()["last_launches"] == 5
+        U1, U2 = st.download(_pad(u).shape)
+    src10 = np.concatenate([src, [0.0]]).astype(np.float32)  # the library adds nothing past its 9 values
+    orc.run_fast(coef, u, v, roc2=roc2, nt=10, src_idx=pt, src=src10)
+    np.testing.assert_array_equal(U1[:, :, :shp[2]], u)
+    np.testing.assert_array_equal(U2[:, :, :shp[2]], v)
 
 
 @pytest.mark.parametrize("kernel,depth", [(1, 2), (1, 3), (1, 5), (1, 6), (5, 2), (5, 4), (0, 2)])
