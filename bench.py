@@ -325,7 +325,8 @@ def extra_adjoint(n=400, nt=2000, ckpt=50, nfd=20, seed=7):
     a +3% Gaussian-blob velocity perturbation; FP32 shot with checkpoints every
     `ckpt` steps through torch.autograd, checked against the FP64 GPU adjoint
     (bit-identical to the FP64 CPU adjoint at test sizes) and against central
-    finite differences of the FP64 forward model on `nfd` cells the wave reached."""
+    finite differences of the FP64 forward model on `nfd` cells the wave reached;
+    the same for the mixed-precision shot (FP64 forward, FP32 backward)."""
     import numpy as np
     import torch
     from girih_amd import adjoint, gpu, physics
@@ -344,9 +345,8 @@ def extra_adjoint(n=400, nt=2000, ckpt=50, nfd=20, seed=7):
         return np.ascontiguousarray(dth ** 2 / r2)
 
     m_obs, m0 = slowness2(0.03), slowness2(0.0)
-    s64 = adjoint.Shot(n, n, n, nt, dth, src_pt, src, rec_k, ckpt=ckpt, dtype=gpu.GR_F64)
-    s32 = adjoint.Shot(n, n, n, nt, dth, src_pt, src.astype(np.float32).astype(np.float64), rec_k, ckpt=ckpt,
-                       dtype=gpu.GR_F32)
+    mixed = adjoint.MixedShot(n, n, n, nt, dth, src_pt, src, rec_k, ckpt=ckpt)
+    s64, s32 = mixed.s64, mixed.s32  # its FP64 and FP32 shots serve the single-precision legs too
     try:
         d_obs = s64.forward(torch.tensor(m_obs, device="cuda"))
         m64 = torch.tensor(m0, device="cuda")
@@ -365,6 +365,20 @@ def extra_adjoint(n=400, nt=2000, ckpt=50, nfd=20, seed=7):
         t2 = time.perf_counter()
         g32 = m32.grad.double()
         rel = float(torch.linalg.norm(g32 - g64) / torch.linalg.norm(g64))
+        # mixed precision: FP64 forward (traces, residual), FP32 backward from its checkpoints
+        mm32 = torch.tensor(m0.astype(np.float32), device="cuda", requires_grad=True)
+        _sync()
+        t3 = time.perf_counter()
+        trm = adjoint.shot_traces(mm32, mixed)
+        _sync()
+        t4 = time.perf_counter()
+        lossm = 0.5 * ((trm - d_obs) ** 2).sum()
+        lossm.backward()
+        _sync()
+        t5 = time.perf_counter()
+        gmix = mm32.grad.double()
+        rel_mix = float(torch.linalg.norm(gmix - g64) / torch.linalg.norm(g64))
+        del trm
         J0 = float(0.5 * ((tr64 - d_obs) ** 2).sum())
         # the FP32 gradient error follows from the FP32 traces' error relative to the residual
         tr_err = float(torch.linalg.norm(tr32.detach().double() - tr64) / torch.linalg.norm(tr64))
@@ -372,10 +386,11 @@ def extra_adjoint(n=400, nt=2000, ckpt=50, nfd=20, seed=7):
         del tr32, tr64
         g = g64.cpu().numpy()
         g32h = g32.cpu().numpy()
+        gmh = gmix.cpu().numpy()
         gmax = np.abs(g).max()
         live = np.argwhere(np.abs(g) > 1e-2 * gmax)
         rng = np.random.default_rng(seed)
-        errs32, errs64, errs32n = [], [], []
+        errs32, errs64, errs32n, errsm = [], [], [], []
 
         def J(mm):
             t = s64.forward(torch.tensor(mm, device="cuda"))
@@ -391,9 +406,9 @@ def extra_adjoint(n=400, nt=2000, ckpt=50, nfd=20, seed=7):
             errs32.append(abs(g32h[k, j, i] - fd) / abs(fd))
             errs64.append(abs(g[k, j, i] - fd) / abs(fd))
             errs32n.append(abs(g32h[k, j, i] - fd) / gmax)
+            errsm.append(abs(gmh[k, j, i] - fd) / abs(fd))
     finally:
-        s32.close()
-        s64.close()
+        mixed.close()
     fwd_glups = n ** 3 * nt / (t1 - t0) / 1e9
     return {"grid": n, "steps": nt, "checkpoint_every": ckpt, "forward_s": round(t1 - t0, 3),
             "backward_s": round(t2 - t1, 3), "forward_glups": round(fwd_glups, 2),
@@ -401,7 +416,11 @@ def extra_adjoint(n=400, nt=2000, ckpt=50, nfd=20, seed=7):
             "rel_l2_grad_fp32_vs_fp64": rel, "fd_cells": len(errs32),
             "fd_max_rel_err_fp32_adjoint": max(errs32), "fd_max_rel_err_fp64_adjoint": max(errs64),
             "fd_max_err_fp32_adjoint_over_max_grad": max(errs32n),
-            "rel_l2_traces_fp32_vs_fp64": tr_err, "residual_over_traces": res_rel}
+            "rel_l2_traces_fp32_vs_fp64": tr_err, "residual_over_traces": res_rel,
+            "mixed": {"forward_s": round(t4 - t3, 3), "backward_s": round(t5 - t4, 3),
+                      "fd_max_rel_err": max(errsm), "rel_l2_grad_vs_fp64": rel_mix,
+                      "note": "FP64 forward (traces, residual); FP32 recomputation and adjoint from the FP64 "
+                              "checkpoints rounded to FP32"}}
 
 
 def main():

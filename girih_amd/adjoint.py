@@ -11,6 +11,11 @@ rec_k after every application).  The forward and the vector-Jacobian product run
 in libgirih_gpu.so (gr_shot_*): the same HIP sweep kernels as the stepper,
 checkpoints every `ckpt` applications, recomputation in the backward pass.  There
 is no CPU path: without the library every call raises.
+
+MixedShot takes an FP32 m and returns FP64 traces: the forward run is FP64, so a
+misfit residual that is a small fraction of the traces keeps its accuracy, and
+the backward pass (recomputation from the FP64 checkpoints rounded to FP32, the
+adjoint and the gradient) runs in FP32.
 """
 import ctypes
 
@@ -81,6 +86,58 @@ class Shot:
         return grad
 
 
+class MixedShot:
+    """FP64 forward run, FP32 backward pass (gr_shot_take_checkpoints).
+
+    forward(m) takes the squared slowness as FP32 [nz, ny, nx] and returns FP64
+    traces [nt, ny, nx]; vjp(m, res) takes FP64 (or FP32) cotangents and returns the
+    FP32 gradient.  The FP32 shot uses the source values rounded to FP32."""
+
+    def __init__(self, nx, ny, nz, nt, dt_over_h, src_point, src, rec_k, ckpt=50, coef=None, tune=True):
+        import numpy as np
+        src = np.asarray(src, dtype=np.float64)
+        self.nx, self.ny, self.nz, self.nt = nx, ny, nz, nt
+        self.dt_over_h = float(dt_over_h)
+        self.s64 = Shot(nx, ny, nz, nt, dt_over_h, src_point, src, rec_k, ckpt=ckpt, dtype=gpu.GR_F64, coef=coef,
+                        tune=tune)
+        self.s32 = Shot(nx, ny, nz, nt, dt_over_h, src_point, src.astype(np.float32).astype(np.float64), rec_k,
+                        ckpt=ckpt, dtype=gpu.GR_F32, coef=coef, tune=tune)
+        self._m_version = None
+
+    def close(self):
+        self.s64.close()
+        self.s32.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def forward(self, m):
+        """FP64 traces for the FP32 squared slowness m (converted exactly to FP64)."""
+        self.s32._check(m, (self.nz, self.ny, self.nx))
+        traces = self.s64.forward(m.double())
+        torch.cuda.current_stream(m.device).synchronize()
+        gpu.check(gpu.lib().gr_set_roc2_from_slowness2(self.s32.stepper._h, ctypes.c_void_p(m.data_ptr()),
+                                                       self.dt_over_h))
+        gpu.check(gpu.lib().gr_shot_take_checkpoints(self.s32._s, self.s64._s))
+        self._m_version = (m.data_ptr(), m._version)
+        return traces
+
+    def vjp(self, m, res):
+        """FP32 gradient [nz, ny, nx] for cotangents res [nt, ny, nx]."""
+        if self._m_version != (m.data_ptr(), m._version):
+            self.forward(m)
+        res = res.to(torch.float32).contiguous()
+        self.s32._check(res, (self.nt, self.ny, self.nx))
+        grad = torch.empty((self.nz, self.ny, self.nx), dtype=torch.float32, device=res.device)
+        torch.cuda.current_stream(res.device).synchronize()
+        gpu.check(gpu.lib().gr_shot_adjoint(self.s32._s, ctypes.c_void_p(res.data_ptr()),
+                                            ctypes.c_void_p(grad.data_ptr())))
+        return grad
+
+
 class _ShotTraces(torch.autograd.Function):
     @staticmethod
     def forward(ctx, m, shot):
@@ -95,5 +152,6 @@ class _ShotTraces(torch.autograd.Function):
 
 
 def shot_traces(m, shot):
-    """Differentiable receiver traces of `shot` for squared slowness m."""
+    """Differentiable receiver traces of `shot` (a Shot or a MixedShot) for squared
+    slowness m."""
     return _ShotTraces.apply(m, shot)

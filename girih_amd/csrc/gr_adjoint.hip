@@ -82,6 +82,18 @@ __global__ void grad_final_kernel(const T *g, T *out, long long n, double scale)
   if (t < n) out[t] = (T)(scale * (double)g[t]);
 }
 
+// dst <- src cell by cell over the whole padded grid (both fields of one grid size;
+// pitches differ with the element size), rounding to nearest when D is narrower
+template <typename S, typename D>
+__global__ void convert_field_kernel(D *dst, const S *src, int nnx, long long spitch, long long dpitch,
+                                     long long total) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= total) return;
+  const long long row = t / nnx;
+  const int i = (int)(t - row * nnx);
+  dst[row * dpitch + i] = (D)src[row * spitch + i];
+}
+
 template <typename T>
 __global__ void zero_kernel(T *g, long long n) {
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -288,6 +300,41 @@ extern "C" int gr_shot_forward(gr_shot *s, void *rec_dev) {
   if (e == hipSuccess) e = hipStreamSynchronize(st);
   if (e != hipSuccess) return shot_fail(e, "gr_shot_forward");
   s->have_forward = true;
+  return GR_OK;
+}
+
+extern "C" int gr_shot_take_checkpoints(gr_shot *dst, const gr_shot *src) {
+  if (!dst || !src) return gr::set_error(GR_EINVAL, "gr_shot_take_checkpoints: null argument");
+  if (!src->have_forward) return gr::set_error(GR_EINVAL, "gr_shot_take_checkpoints: the source shot has no forward run");
+  const Problem &D = gr_internal_problem(dst->h), &S = gr_internal_problem(src->h);
+  if (D.nx != S.nx || D.ny != S.ny || D.nz != S.nz || dst->nt != src->nt || dst->ckpt != src->ckpt ||
+      dst->rec_k != src->rec_k)
+    return gr::set_error(GR_EINVAL, "gr_shot_take_checkpoints: the shots differ in grid, nt, ckpt or receiver plane");
+  if (S.dtype == GR_F32 && D.dtype == GR_F64)
+    return gr::set_error(GR_EUNSUPPORTED, "gr_shot_take_checkpoints: FP32 checkpoints cannot seed an FP64 shot");
+  // the source shot's stream has finished its forward run (gr_shot_forward synchronises)
+  hipStream_t st = gr_internal_stream(dst->h);
+  const int nnx = D.nx + 2 * D.R;
+  const long long total = (long long)nnx * D.rows * D.planes;
+  const unsigned nb = (unsigned)((total + 255) / 256);
+  hipError_t e = hipSuccess;
+  for (size_t i = 0; i < dst->ck.size() && e == hipSuccess; ++i) {
+    void *d = fbase(D, dst->ck[i]);
+    const void *sp = fbase(S, src->ck[i]);
+    if (S.dtype == GR_F64 && D.dtype == GR_F64)
+      hipLaunchKernelGGL((convert_field_kernel<double, double>), dim3(nb), dim3(256), 0, st, (double *)d,
+                         (const double *)sp, nnx, (long long)S.pitch, (long long)D.pitch, total);
+    else if (S.dtype == GR_F64)
+      hipLaunchKernelGGL((convert_field_kernel<double, float>), dim3(nb), dim3(256), 0, st, (float *)d,
+                         (const double *)sp, nnx, (long long)S.pitch, (long long)D.pitch, total);
+    else
+      hipLaunchKernelGGL((convert_field_kernel<float, float>), dim3(nb), dim3(256), 0, st, (float *)d,
+                         (const float *)sp, nnx, (long long)S.pitch, (long long)D.pitch, total);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  if (e != hipSuccess) return shot_fail(e, "gr_shot_take_checkpoints");
+  dst->have_forward = true;
   return GR_OK;
 }
 

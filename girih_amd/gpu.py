@@ -73,7 +73,7 @@ EXPORTS = ["gr_create", "gr_destroy", "gr_last_error", "gr_upload", "gr_download
            "gr_init_roc2_model",
            "gr_download_roc2", "gr_batch_create", "gr_batch_destroy", "gr_batch_problem", "gr_batch_set_config",
            "gr_batch_step", "gr_batch_set_time_block", "gr_batch_autotune", "gr_batch_get_stats", "gr_shot_create", "gr_shot_destroy",
-           "gr_set_roc2_from_slowness2", "gr_shot_forward", "gr_shot_adjoint"]
+           "gr_set_roc2_from_slowness2", "gr_shot_forward", "gr_shot_adjoint", "gr_shot_take_checkpoints"]
 
 _lib = None
 
@@ -131,6 +131,7 @@ def lib():
             "gr_set_roc2_from_slowness2": ([P, P, ctypes.c_double], I),
             "gr_shot_forward": ([P, P], I),
             "gr_shot_adjoint": ([P, P, P], I),
+            "gr_shot_take_checkpoints": ([P, P], I),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)

@@ -285,6 +285,12 @@ int gr_set_roc2_from_slowness2(gr_handle *h, const void *m_dev, double dt_over_h
 int gr_shot_forward(gr_shot *s, void *trace_dev);
 /* Vector-Jacobian product of the last forward run: grad = sum_a res[a] . d trace[a] / dm. */
 int gr_shot_adjoint(gr_shot *s, const void *res_dev, void *grad_dev);
+/* Give dst the checkpoints of src's last forward run (same grid, nt, ckpt and
+ * receiver plane), rounded to dst's field type: an FP64 forward run can seed an
+ * FP32 adjoint (mixed precision: FP64 traces and residuals, FP32 recomputation and
+ * adjoint), or an FP32/FP64 shot its own type's copy.  FP32 -> FP64 returns
+ * GR_EUNSUPPORTED.  dst's roc2 must be set for the same m before gr_shot_adjoint. */
+int gr_shot_take_checkpoints(gr_shot *dst, const gr_shot *src);
 
 #ifdef __cplusplus
 }

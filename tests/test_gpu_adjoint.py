@@ -5,11 +5,17 @@
   not divide nt;
 * FP32 (GR_FAST): gradient within 1e-4 relative L2 of the FP64 CPU gradient, and
   independent of the checkpoint interval (bit for bit);
+* mixed precision (MixedShot: FP64 forward, FP32 backward from the FP64 checkpoints
+  rounded by gr_shot_take_checkpoints): traces bit-identical to the FP64 CPU shot,
+  gradient within 1e-5 relative L2 of the FP64 CPU gradient; a hand-off between shots
+  of one type gives the source shot's own gradient bit for bit;
 * finite differences of the FP64 GPU forward model on random cells the wave has
   reached: on the small shot both gradients (FP32 through torch.autograd) within 1e-3
-  per cell; on the 400^3 x 2000-step shot the FP64 gradient within 1e-3 and the FP32
-  gradient within its measured bound FP32_FD_BOUND_400.
+  per cell; on the 400^3 x 2000-step shot the FP64 and the mixed-precision gradients
+  within 1e-3 and the FP32 gradient within its measured bound FP32_FD_BOUND_400.
 """
+import ctypes
+
 import numpy as np
 import pytest
 import torch
@@ -105,6 +111,76 @@ def test_fp32_gradient_vs_fp64_cpu_and_checkpoint_independence():
     assert rel < 1e-4
 
 
+def test_mixed_shot_vs_fp64_cpu_adjoint():
+    """FP64 traces from an FP32 m (exactly widened), and the FP32 backward pass seeded
+    by the FP64 checkpoints, against the FP64 CPU shot and its gradient."""
+    m = _m(3).astype(np.float32)
+    m64 = m.astype(np.float64)
+    obs = orc.shot(physics.fd8_coef(), _roc2_full(_m(4)), NT, SRC_PT, _src(), REC_K)[0]
+    tr_cpu = orc.shot(physics.fd8_coef(), _roc2_full(m64), NT, SRC_PT, _src(), REC_K)[0]
+    res64 = tr_cpu - obs
+    _, g_cpu = orc.shot(physics.fd8_coef(), _roc2_full(m64), NT, SRC_PT, _src(), REC_K, res=res64,
+                        dt_over_h=DT_OVER_H)
+    shot = adjoint.MixedShot(NX, NY, NZ, NT, DT_OVER_H, SRC_PT, _src(), REC_K, ckpt=7)
+    try:
+        mt = torch.tensor(m, device="cuda", requires_grad=True)
+        tr = adjoint.shot_traces(mt, shot)
+        assert tr.dtype == torch.float64
+        np.testing.assert_array_equal(tr.detach().cpu().numpy(), tr_cpu)
+        loss = 0.5 * ((tr - torch.tensor(obs, device="cuda")) ** 2).sum()
+        loss.backward()
+        g = mt.grad
+        assert g.dtype == torch.float32
+        direct = shot.vjp(mt.detach(), torch.tensor(res64, device="cuda"))
+        np.testing.assert_array_equal(g.cpu().numpy(), direct.cpu().numpy())
+    finally:
+        shot.close()
+    rel = np.linalg.norm(g.double().cpu().numpy() - g_cpu) / np.linalg.norm(g_cpu)
+    print("mixed-precision gradient rel L2 vs FP64 CPU", rel, flush=True)
+    assert np.abs(g_cpu).max() > 0
+    assert rel < 1e-5
+
+
+@pytest.mark.parametrize("dtype", [gpu.GR_F32, gpu.GR_F64])
+def test_take_checkpoints_same_type_bitexact(dtype):
+    """A shot seeded with another shot's checkpoints of the same type (and the same m)
+    gives that shot's gradient bit for bit."""
+    m = _m(6)
+    res = np.random.default_rng(8).standard_normal((NT, NY, NX))
+    tdt = torch.float32 if dtype == gpu.GR_F32 else torch.float64
+    a, b = _shot(dtype, ckpt=6), _shot(dtype, ckpt=6)
+    try:
+        mt = torch.tensor(m, dtype=tdt, device="cuda")
+        rt = torch.tensor(res, dtype=tdt, device="cuda")
+        a.forward(mt)
+        ga = a.vjp(mt, rt)
+        gpu.check(gpu.lib().gr_set_roc2_from_slowness2(b.stepper._h, ctypes.c_void_p(mt.data_ptr()), DT_OVER_H))
+        gpu.check(gpu.lib().gr_shot_take_checkpoints(b._s, a._s))
+        gb = torch.empty_like(ga)
+        gpu.check(gpu.lib().gr_shot_adjoint(b._s, ctypes.c_void_p(rt.data_ptr()), ctypes.c_void_p(gb.data_ptr())))
+        np.testing.assert_array_equal(ga.cpu().numpy(), gb.cpu().numpy())
+    finally:
+        a.close()
+        b.close()
+
+
+def test_take_checkpoints_errors():
+    s64, s32 = _shot(gpu.GR_F64, ckpt=5), _shot(gpu.GR_F32, ckpt=5)
+    other = adjoint.Shot(NX, NY, NZ, NT, DT_OVER_H, SRC_PT, _src(), REC_K, ckpt=4, dtype=gpu.GR_F32)
+    try:
+        lib = gpu.lib()
+        assert lib.gr_shot_take_checkpoints(s32._s, s64._s) == gpu.GR_EINVAL  # no forward run yet
+        s64.forward(torch.tensor(_m(1), device="cuda"))
+        s32.forward(torch.tensor(_m(1), dtype=torch.float32, device="cuda"))
+        assert lib.gr_shot_take_checkpoints(s64._s, s32._s) == gpu.GR_EUNSUPPORTED  # FP32 -> FP64
+        assert lib.gr_shot_take_checkpoints(other._s, s64._s) == gpu.GR_EINVAL  # another checkpoint interval
+        assert lib.gr_shot_take_checkpoints(None, s64._s) == gpu.GR_EINVAL
+        assert lib.gr_shot_take_checkpoints(s32._s, s64._s) == gpu.GR_OK
+    finally:
+        for s in (s64, s32, other):
+            s.close()
+
+
 # Per-cell relative error of the FP32 gradient against central differences of the FP64
 # misfit at the 400^3 x 2000-step shot, asserted as measured there (6.5e-3): the FP32
 # forward model's traces differ from FP64 by storage and Laplacian rounding (about 3e-6
@@ -167,16 +243,17 @@ def test_gradients_vs_finite_differences_and_autograd():
 def test_full_size_shot_gradient_bounds():
     """The 400^3 x 2000-step shot (layered 8-layer velocity, +3% blob for the observed
     data, 15 Hz Ricker at 50 m depth, receivers at z = 8 cells, checkpoints every 50):
-    the FP64 adjoint gradient meets 1e-3 per cell against central finite differences of
-    the FP64 misfit, and the FP32 gradient meets the bound FP32_FD_BOUND_400 measured
-    for it (10 cells each)."""
+    the FP64 and the mixed-precision adjoint gradients meet 1e-3 per cell against
+    central finite differences of the FP64 misfit, and the FP32 gradient meets the bound
+    FP32_FD_BOUND_400 measured for it (10 cells each)."""
     import bench
 
     out = bench.extra_adjoint(nfd=10)
     print({k: out[k] for k in ("fd_max_rel_err_fp64_adjoint", "fd_max_rel_err_fp32_adjoint",
-                               "rel_l2_grad_fp32_vs_fp64", "forward_s", "backward_s")}, flush=True)
+                               "rel_l2_grad_fp32_vs_fp64", "forward_s", "backward_s", "mixed")}, flush=True)
     assert out["fd_cells"] == 10
     assert out["fd_max_rel_err_fp64_adjoint"] <= 1e-3
+    assert out["mixed"]["fd_max_rel_err"] <= 1e-3
     assert out["fd_max_rel_err_fp32_adjoint"] <= FP32_FD_BOUND_400
 
 
