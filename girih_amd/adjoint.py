@@ -12,10 +12,10 @@ in libgirih_gpu.so (gr_shot_*): the same HIP sweep kernels as the stepper,
 checkpoints every `ckpt` applications, recomputation in the backward pass.  There
 is no CPU path: without the library every call raises.
 
-MixedShot takes an FP32 m and returns FP64 traces: the forward run is FP64, so a
-misfit residual that is a small fraction of the traces keeps its accuracy, and
-the backward pass (recomputation from the FP64 checkpoints rounded to FP32, the
-adjoint and the gradient) runs in FP32.
+MixedShot returns FP64 traces: the forward run is FP64, so a misfit residual that
+is a small fraction of the traces keeps its accuracy, and the backward pass
+(recomputation from the FP64 checkpoints rounded to FP32, the adjoint and the
+gradient) runs in FP32.  m may be FP64 (the forward run uses it as it is) or FP32.
 """
 import ctypes
 
@@ -89,9 +89,10 @@ class Shot:
 class MixedShot:
     """FP64 forward run, FP32 backward pass (gr_shot_take_checkpoints).
 
-    forward(m) takes the squared slowness as FP32 [nz, ny, nx] and returns FP64
-    traces [nt, ny, nx]; vjp(m, res) takes FP64 (or FP32) cotangents and returns the
-    FP32 gradient.  The FP32 shot uses the source values rounded to FP32."""
+    forward(m) takes the squared slowness [nz, ny, nx] as FP64 or FP32 and returns
+    FP64 traces [nt, ny, nx]; vjp(m, res) takes FP64 (or FP32) cotangents and returns
+    the gradient, computed in FP32, in m's dtype.  The FP32 shot's roc2 comes from m
+    rounded to FP32 and its source values are rounded to FP32."""
 
     def __init__(self, nx, ny, nz, nt, dt_over_h, src_point, src, rec_k, ckpt=50, coef=None, tune=True):
         import numpy as np
@@ -115,19 +116,23 @@ class MixedShot:
             pass
 
     def forward(self, m):
-        """FP64 traces for the FP32 squared slowness m (converted exactly to FP64)."""
-        self.s32._check(m, (self.nz, self.ny, self.nx))
-        traces = self.s64.forward(m.double())
+        """FP64 traces for the squared slowness m (FP64, or FP32 widened exactly)."""
+        if m.dtype not in (torch.float32, torch.float64):
+            raise ValueError("expected an FP32 or FP64 tensor")
+        m64 = m.double()  # m itself when it is FP64
+        self.s64._check(m64, (self.nz, self.ny, self.nx))
+        traces = self.s64.forward(m64)
+        m32 = m.float().contiguous()
         torch.cuda.current_stream(m.device).synchronize()
-        gpu.check(gpu.lib().gr_set_roc2_from_slowness2(self.s32.stepper._h, ctypes.c_void_p(m.data_ptr()),
+        gpu.check(gpu.lib().gr_set_roc2_from_slowness2(self.s32.stepper._h, ctypes.c_void_p(m32.data_ptr()),
                                                        self.dt_over_h))
         gpu.check(gpu.lib().gr_shot_take_checkpoints(self.s32._s, self.s64._s))
-        self._m_version = (m.data_ptr(), m._version)
+        self._m_version = (m.data_ptr(), m._version, m.dtype)
         return traces
 
     def vjp(self, m, res):
-        """FP32 gradient [nz, ny, nx] for cotangents res [nt, ny, nx]."""
-        if self._m_version != (m.data_ptr(), m._version):
+        """Gradient [nz, ny, nx] in m's dtype (computed in FP32) for cotangents res [nt, ny, nx]."""
+        if self._m_version != (m.data_ptr(), m._version, m.dtype):
             self.forward(m)
         res = res.to(torch.float32).contiguous()
         self.s32._check(res, (self.nt, self.ny, self.nx))
@@ -135,7 +140,7 @@ class MixedShot:
         torch.cuda.current_stream(res.device).synchronize()
         gpu.check(gpu.lib().gr_shot_adjoint(self.s32._s, ctypes.c_void_p(res.data_ptr()),
                                             ctypes.c_void_p(grad.data_ptr())))
-        return grad
+        return grad.to(m.dtype)
 
 
 class _ShotTraces(torch.autograd.Function):

@@ -133,12 +133,23 @@ def test_mixed_shot_vs_fp64_cpu_adjoint():
         assert g.dtype == torch.float32
         direct = shot.vjp(mt.detach(), torch.tensor(res64, device="cuda"))
         np.testing.assert_array_equal(g.cpu().numpy(), direct.cpu().numpy())
+        # an FP64 m: the forward run uses it as it is, the gradient comes back as FP64
+        m2 = _m(5)
+        tr2_cpu = orc.shot(physics.fd8_coef(), _roc2_full(m2), NT, SRC_PT, _src(), REC_K)[0]
+        _, g2_cpu = orc.shot(physics.fd8_coef(), _roc2_full(m2), NT, SRC_PT, _src(), REC_K, res=tr2_cpu - obs,
+                             dt_over_h=DT_OVER_H)
+        mt2 = torch.tensor(m2, device="cuda")
+        tr2 = shot.forward(mt2)
+        np.testing.assert_array_equal(tr2.cpu().numpy(), tr2_cpu)
+        g2 = shot.vjp(mt2, tr2 - torch.tensor(obs, device="cuda"))
+        assert g2.dtype == torch.float64
     finally:
         shot.close()
-    rel = np.linalg.norm(g.double().cpu().numpy() - g_cpu) / np.linalg.norm(g_cpu)
-    print("mixed-precision gradient rel L2 vs FP64 CPU", rel, flush=True)
-    assert np.abs(g_cpu).max() > 0
-    assert rel < 1e-5
+    for gg, ref in ((g.double().cpu().numpy(), g_cpu), (g2.cpu().numpy(), g2_cpu)):
+        rel = np.linalg.norm(gg - ref) / np.linalg.norm(ref)
+        print("mixed-precision gradient rel L2 vs FP64 CPU", rel, flush=True)
+        assert np.abs(ref).max() > 0
+        assert rel < 1e-5
 
 
 @pytest.mark.parametrize("dtype", [gpu.GR_F32, gpu.GR_F64])
