@@ -87,8 +87,7 @@ hipError_t wave2_pass(const Problem &P, const void *X, const void *Y, void *ZO, 
     return w2_dispatch<double, double, false>(rocs, strm, w2_args<double, double>(P, X, Y, ZO, ZN, src, src_idx, lc),
                                               st);
   }
-  if (P.flags & GR_COEF_BF16) {  // a bf16 roc2 field (never scalar)
-    if (rocs) return hipErrorInvalidValue;
+  if ((P.flags & GR_COEF_BF16) && !rocs) {  // a bf16 roc2 field (a scalar roc2 has no storage type)
     const W2Args<float, __hip_bfloat16> a = w2_args<float, __hip_bfloat16>(P, X, Y, ZO, ZN, src, src_idx, lc);
     if (fast) return w2_dispatch<float, __hip_bfloat16, true>(false, strm, a, st);
     return w2_dispatch<float, __hip_bfloat16, false>(false, strm, a, st);

@@ -101,6 +101,30 @@ def test_bf16_roc2_wave_two_step_passes(zchunk):
     np.testing.assert_array_equal(U2[:, :, :nnx], v)
 
 
+def test_scalar_roc2_with_bf16_flag_two_step_passes():
+    """GR_COEF_BF16 with a scalar roc2 has no field to store as bf16: the two-step
+    pass runs the scalar form and matches the oracle's GR_FAST restatement."""
+    nx, ny, nz = 70, 21, 13
+    rng = np.random.default_rng(92)
+    shp = orc.shape_of(0, nx, ny, nz)
+    u = np.zeros(shp, np.float32)
+    v = np.zeros(shp, np.float32)
+    u[4:-4, 4:-4, 4:-4] = rng.uniform(-1, 1, (nz, ny, nx))
+    v[4:-4, 4:-4, 4:-4] = rng.uniform(-1, 1, (nz, ny, nx))
+    w = [-205.0 / 72, 8.0 / 5, -1.0 / 5, 8.0 / 315, -1.0 / 560]
+    coef = [3 * w[0]] + w[1:5]
+    flags = gpu.GR_FAST | gpu.GR_ROC2_SCALAR | gpu.GR_COEF_BF16
+    with gpu.Stepper(0, nx, ny, nz, dtype=gpu.GR_F32, flags=flags, roc2=0.12, coef=coef) as st:
+        st.upload(_pad(u), _pad(v))
+        st.set_config(t_block=2)
+        st.step(4)
+        assert st.stats()["last_launches"] == 2
+        U1, U2 = st.download(_pad(u).shape)
+    orc.run_fast(coef, u, v, roc2s=0.12, nt=4)
+    np.testing.assert_array_equal(U1[:, :, :shp[2]], u)
+    np.testing.assert_array_equal(U2[:, :, :shp[2]], v)
+
+
 def test_bf16_roc2_fast_two_step_bitexact_vs_oracle():
     """GR_FAST | GR_COEF_BF16 (the batch's arithmetic and storage) in two-step
     passes, with a source: bit-identical to the oracle's GR_FAST restatement run on
