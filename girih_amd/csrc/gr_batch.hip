@@ -21,6 +21,7 @@
 // bf16 (GR_COEF_BF16) or FP32.
 #include <hip/hip_runtime.h>
 #include <hip/hip_bf16.h>
+#include <hip/hip_cooperative_groups.h>
 
 #include <algorithm>
 #include <numeric>
@@ -70,13 +71,15 @@ __device__ __forceinline__ X uniform_load(const X *p) {
   return x;
 }
 
+// Claim and run the items of one step until the queue is empty.  `base` is the
+// counter value at which the step's claims start.
 template <typename C, int BX, int BY, int RY, bool RING>
-__global__ __launch_bounds__(64 * BX * BY) void batch_kernel(const BatchDesc *__restrict__ d, int nprob, int nitems,
-                                                             unsigned long long *ctr, unsigned long long base,
-                                                             int step) {
+__device__ __forceinline__ void step_items(const BatchDesc *__restrict__ d, int nprob, int nitems,
+                                           unsigned long long *ctr, unsigned long long base, int step,
+                                           typename SweepTile<float, C, 0, BY, RY, true, false, RING, kFastND, BX,
+                                                              true>::Lds &lds,
+                                           int *sclaim) {
   using ST = SweepTile<float, C, 0, BY, RY, true, false, RING, kFastND, BX, true>;  // a step streams every problem
-  __shared__ typename ST::Lds lds;
-  __shared__ int sclaim[2];
   const bool leader = threadIdx.x == 0 && threadIdx.y == 0;
   for (;;) {
     if (leader) {
@@ -119,6 +122,37 @@ __global__ __launch_bounds__(64 * BX * BY) void batch_kernel(const BatchDesc *__
     ST::run(a, rem - ty * D.tiles_x, ty, zc, lds);
     __syncthreads();  // LDS and the claim slots are reused by the next item
   }
+  __syncthreads();  // every lane has read the failing claim before the slots are reused
+}
+
+// One step per launch.
+template <typename C, int BX, int BY, int RY, bool RING>
+__global__ __launch_bounds__(64 * BX * BY) void batch_kernel(const BatchDesc *__restrict__ d, int nprob, int nitems,
+                                                             unsigned long long *ctr, unsigned long long base,
+                                                             int step) {
+  using ST = SweepTile<float, C, 0, BY, RY, true, false, RING, kFastND, BX, true>;
+  __shared__ typename ST::Lds lds;
+  __shared__ int sclaim[2];
+  step_items<C, BX, BY, RY, RING>(d, nprob, nitems, ctr, base, step, lds, sclaim);
+}
+
+// Every step of a call in one cooperative launch: the workgroups run a step's
+// queue, then meet at a grid barrier.  The barrier's agent-scope release and
+// acquire write back and invalidate each XCD's L2 once per step, the maintenance
+// a launch boundary does, so the next step reads what every XCD wrote.  Step s
+// reads table d + (s & 1) * nprob; its claims start at s * (nitems + gridDim.x).
+template <typename C, int BX, int BY, int RY, bool RING>
+__global__ __launch_bounds__(64 * BX * BY) void batch_run_kernel(const BatchDesc *__restrict__ d, int nprob,
+                                                                 int nitems, unsigned long long *ctr, int napply) {
+  using ST = SweepTile<float, C, 0, BY, RY, true, false, RING, kFastND, BX, true>;
+  __shared__ typename ST::Lds lds;
+  __shared__ int sclaim[2];
+  cooperative_groups::grid_group grid = cooperative_groups::this_grid();
+  const unsigned long long per = (unsigned long long)nitems + gridDim.x;
+  for (int s = 0; s < napply; ++s) {
+    step_items<C, BX, BY, RY, RING>(d + (size_t)(s & 1) * nprob, nprob, nitems, ctr, per * s, s, lds, sclaim);
+    grid.sync();
+  }
 }
 
 // Two-step blocks: one item is (grid, xy tile, z chunk) for two time steps, run on
@@ -137,15 +171,13 @@ static_assert(sizeof(W2BatchDesc<float>) == sizeof(W2BatchDesc<__hip_bfloat16>),
 using W2Tile = Wave2Tile<float, float, true, false, true, 1, 8, 2>;  // 64 x 16 output tiles, 512 lanes
 constexpr int kW2Lanes = W2Tile::NT;
 
-// MINW = 4 caps the registers at 128 so that two workgroups share a CU (a few
-// queue values spill); MINW = 1 keeps every value in registers, one workgroup per CU.
-template <typename C, int MINW>
-__global__ __launch_bounds__(kW2Lanes) __attribute__((amdgpu_waves_per_eu(MINW))) void batch_w2_kernel(
-    const W2BatchDesc<C> *__restrict__ d, int nprob, int nitems, unsigned long long *ctr, unsigned long long base,
-    int pass) {
+// Claim and run the items of one two-step block until the queue is empty.
+template <typename C>
+__device__ __forceinline__ void block_items(const W2BatchDesc<C> *__restrict__ d, int nprob, int nitems,
+                                            unsigned long long *ctr, unsigned long long base, int pass,
+                                            typename Wave2Tile<float, C, true, false, true, 1, 8, 2>::Lds &lds,
+                                            int *sclaim) {
   using WT = Wave2Tile<float, C, true, false, true, 1, 8, 2>;  // a block streams every problem
-  __shared__ typename WT::Lds lds;
-  __shared__ int sclaim[2];
   const bool leader = threadIdx.x == 0;
   for (;;) {
     if (leader) {
@@ -170,10 +202,41 @@ __global__ __launch_bounds__(kW2Lanes) __attribute__((amdgpu_waves_per_eu(MINW))
     WT::run(a, local % ntile, local / ntile, lds);
     __syncthreads();  // LDS and the claim slots are reused by the next item
   }
+  __syncthreads();  // every lane has read the failing claim before the slots are reused
+}
+
+// MINW = 4 caps the registers at 128 so that two workgroups share a CU (a few
+// queue values spill); MINW = 1 keeps every value in registers, one workgroup per CU.
+template <typename C, int MINW>
+__global__ __launch_bounds__(kW2Lanes) __attribute__((amdgpu_waves_per_eu(MINW))) void batch_w2_kernel(
+    const W2BatchDesc<C> *__restrict__ d, int nprob, int nitems, unsigned long long *ctr, unsigned long long base,
+    int pass) {
+  __shared__ typename Wave2Tile<float, C, true, false, true, 1, 8, 2>::Lds lds;
+  __shared__ int sclaim[2];
+  block_items<C>(d, nprob, nitems, ctr, base, pass, lds, sclaim);
+}
+
+// Every two-step block of a call in one cooperative launch (grid barrier between
+// blocks, as batch_run_kernel).  Block p reads table 0 (p = 0), 1 (odd p) or 2
+// (even p > 0) of d, or table 3 for every block when tuning.
+template <typename C, int MINW>
+__global__ __launch_bounds__(kW2Lanes) __attribute__((amdgpu_waves_per_eu(MINW))) void batch_w2_run_kernel(
+    const W2BatchDesc<C> *__restrict__ d, int nprob, int nitems, unsigned long long *ctr, int npasses, int tuning) {
+  __shared__ typename Wave2Tile<float, C, true, false, true, 1, 8, 2>::Lds lds;
+  __shared__ int sclaim[2];
+  cooperative_groups::grid_group grid = cooperative_groups::this_grid();
+  const unsigned long long per = (unsigned long long)nitems + gridDim.x;
+  for (int p = 0; p < npasses; ++p) {
+    const int t = tuning ? 3 : p == 0 ? 0 : (p & 1) ? 1 : 2;
+    block_items<C>(d + (size_t)t * nprob, nprob, nitems, ctr, per * p, p, lds, sclaim);
+    grid.sync();
+  }
 }
 
 typedef void (*batch_fn)(const BatchDesc *, int, int, unsigned long long *, unsigned long long, int);
+typedef void (*batch_run_fn)(const BatchDesc *, int, int, unsigned long long *, int);
 typedef void (*w2_fn)(const W2BatchDesc<float> *, int, int, unsigned long long *, unsigned long long, int);
+typedef void (*w2_run_fn)(const W2BatchDesc<float> *, int, int, unsigned long long *, int, int);
 
 // two-step kernels by register cap; the bf16 instantiation reads the same table
 // bytes (W2BatchDesc<bf16> and W2BatchDesc<float> differ in the roc2 pointer's type only)
@@ -185,16 +248,26 @@ w2_fn w2_kernel(bool bf16, int v) {
              : reinterpret_cast<w2_fn>(&batch_w2_kernel<__hip_bfloat16, 1>);
   return v ? &batch_w2_kernel<float, 4> : &batch_w2_kernel<float, 1>;
 }
+w2_run_fn w2_run_kernel(bool bf16, int v) {
+  if (bf16)
+    return v ? reinterpret_cast<w2_run_fn>(&batch_w2_run_kernel<__hip_bfloat16, 4>)
+             : reinterpret_cast<w2_run_fn>(&batch_w2_run_kernel<__hip_bfloat16, 1>);
+  return v ? &batch_w2_run_kernel<float, 4> : &batch_w2_run_kernel<float, 1>;
+}
 
 struct BatchVariant {
   int bx, by, ry;
   bool ring;
-  batch_fn f32, bf16;
+  batch_fn f32, bf16;          // one step per launch
+  batch_run_fn run32, runbf16;  // every step of a call in one cooperative launch
 };
 
 // tiles of 64*BX x BY*RY cells, 64*BX x BY lanes
-#define GR_BV(BX, BY, RY, RING) \
-  { BX, BY, RY, RING, &batch_kernel<float, BX, BY, RY, RING>, &batch_kernel<__hip_bfloat16, BX, BY, RY, RING> }
+#define GR_BV(BX, BY, RY, RING)                                                                                   \
+  {                                                                                                              \
+    BX, BY, RY, RING, &batch_kernel<float, BX, BY, RY, RING>, &batch_kernel<__hip_bfloat16, BX, BY, RY, RING>,   \
+        &batch_run_kernel<float, BX, BY, RY, RING>, &batch_run_kernel<__hip_bfloat16, BX, BY, RY, RING>         \
+  }
 const BatchVariant kVariants[] = {GR_BV(1, 8, 4, false), GR_BV(1, 8, 4, true), GR_BV(1, 4, 4, false),
                                   GR_BV(1, 8, 2, true),  GR_BV(1, 4, 4, true), GR_BV(2, 8, 2, true),
                                   GR_BV(2, 4, 4, true)};
@@ -225,6 +298,11 @@ struct gr_batch {
   int w2_variant = 0;  // index into kW2Minw
   int w2_items = 0, w2_grid = 0, w2_occupancy[kNumW2] = {};
   int last_tblock = 1;  // steps per launch of the last call's blocks
+  // one cooperative launch per call (gr_batch_step mode 2)
+  bool coop = false;                                          // the device supports cooperative launches
+  int run_occupancy[kNumVariants] = {}, w2_run_occupancy[kNumW2] = {};
+  int run_grid = 0, w2_run_grid = 0;
+  int last_mode = 0, last_grid = 0;
   unsigned long long *ctr = nullptr;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -307,6 +385,7 @@ int build_tables(gr_batch *b, bool spares, int npasses = 0) {
   }
   b->nitems = item;
   b->grid = std::max(1, std::min(item, b->ncu * std::max(1, b->occupancy[b->variant])));
+  b->run_grid = std::max(1, std::min(item, b->ncu * std::max(1, b->run_occupancy[b->variant])));
   const size_t bytes = sizeof(BatchDesc) * (spares ? 4 : 2) * (size_t)n;
   hipError_t e = hipMemcpyAsync(b->ddesc, b->hdesc.data(), bytes, hipMemcpyHostToDevice, b->stream);
   if (e != hipSuccess) return set_errorf(GR_EHIP, "gr_batch: descriptor copy: %s", hipGetErrorString(e));
@@ -368,6 +447,7 @@ int build_w2_tables(gr_batch *b) {
   }
   b->w2_items = item;
   b->w2_grid = std::max(1, std::min(item, b->ncu * std::max(1, b->w2_occupancy[b->w2_variant])));
+  b->w2_run_grid = std::max(1, std::min(item, b->ncu * std::max(1, b->w2_run_occupancy[b->w2_variant])));
   hipError_t e = hipMemcpyAsync(b->dw2, b->hw2.data(), sizeof(W2BatchDesc<float>) * b->hw2.size(),
                                 hipMemcpyHostToDevice, b->stream);
   if (e != hipSuccess) return set_errorf(GR_EHIP, "gr_batch: descriptor copy: %s", hipGetErrorString(e));
@@ -404,17 +484,51 @@ hipError_t enqueue_steps(gr_batch *b, int table, int napply, int tblock, bool tu
   return hipGetLastError();
 }
 
+// enqueue_steps' work as one cooperative launch: every single step of the call, or
+// every two-step block (then, for an odd count, one single-step launch whose claims
+// start after the blocks').  Never captured into a graph.
+hipError_t enqueue_run(gr_batch *b, int table, int napply, int tblock, bool tuning, hipStream_t st) {
+  int n = (int)b->probs.size();
+  const BatchVariant &V = kVariants[b->variant];
+  unsigned long long *ctr = b->ctr;
+  hipLaunchKernelGGL(batch_reset_kernel, dim3(1), dim3(64), 0, st, b->ctr);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  if (tblock != 2) {
+    const BatchDesc *tab = b->ddesc + (size_t)table * n;
+    int items = b->nitems;
+    void *args[] = {&tab, &n, &items, &ctr, &napply};
+    const void *f = reinterpret_cast<const void *>(b->bf16 ? V.runbf16 : V.run32);
+    return hipLaunchCooperativeKernel(f, dim3(b->run_grid), dim3(64 * V.bx, V.by), args, 0, st);
+  }
+  int npasses = napply / 2, tun = tuning ? 1 : 0, items = b->w2_items;
+  const W2BatchDesc<float> *tab = b->dw2;
+  void *args[] = {&tab, &n, &items, &ctr, &npasses, &tun};
+  const void *f = reinterpret_cast<const void *>(w2_run_kernel(b->bf16, b->w2_variant));
+  e = hipLaunchCooperativeKernel(f, dim3(b->w2_run_grid), dim3(kW2Lanes), args, 0, st);
+  if (e != hipSuccess || !(napply & 1)) return e;
+  const unsigned long long base = (unsigned long long)npasses * (b->w2_items + b->w2_run_grid);
+  hipLaunchKernelGGL(b->bf16 ? V.bf16 : V.f32, dim3(b->grid), dim3(64 * V.bx, V.by), 0, st, b->ddesc, n, b->nitems,
+                     b->ctr, base, 2 * npasses);
+  return hipGetLastError();
+}
+
 void drop_graph(gr_batch *b) {
   if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
   b->gexec = nullptr;
   b->g_napply = b->g_variant = b->g_zchunk = b->g_table = b->g_tblock = b->g_w2_zchunk = b->g_w2_variant = -1;
 }
 
-// Run one call (enqueue_steps' arguments), timed, optionally replayed from a graph.
-int run_steps(gr_batch *b, int table, int napply, bool graph, int tblock = 1, bool tuning = false) {
+// Run one call (enqueue_steps' arguments), timed: mode 0 launches per step (or
+// block), 1 replays them from a graph, 2 runs them as one cooperative launch.
+int run_steps(gr_batch *b, int table, int napply, int mode, int tblock = 1, bool tuning = false) {
   hipError_t e = hipSuccess;
   const int gtable = tuning ? 3 : table;
-  if (graph && napply > 0) {
+  const bool graph = mode == 1;
+  if (mode == 2 && napply > 0) {
+    e = hipEventRecord(b->ev0, b->stream);
+    if (e == hipSuccess) e = enqueue_run(b, table, napply, tblock, tuning, b->stream);
+  } else if (graph && napply > 0) {
     if (!b->gexec || b->g_napply != napply || b->g_variant != b->variant || b->g_zchunk != b->zchunk ||
         b->g_table != gtable || b->g_tblock != tblock || b->g_w2_zchunk != b->w2_zchunk ||
         b->g_w2_variant != b->w2_variant) {
@@ -453,7 +567,14 @@ int run_steps(gr_batch *b, int table, int napply, bool graph, int tblock = 1, bo
   (void)hipEventElapsedTime(&ms, b->ev0, b->ev1);
   b->last_ms = ms;
   b->last_tblock = tblock;
-  b->last_launches = tblock == 2 ? napply / 2 + (napply & 1) : napply;
+  b->last_mode = mode;
+  if (mode == 2) {
+    b->last_launches = tblock == 2 ? (napply >= 2) + (napply & 1) : (napply > 0);
+    b->last_grid = tblock == 2 ? b->w2_run_grid : b->run_grid;
+  } else {
+    b->last_launches = tblock == 2 ? napply / 2 + (napply & 1) : napply;
+    b->last_grid = tblock == 2 ? b->w2_grid : b->grid;
+  }
   long long up = 0;
   for (gr_handle *h : b->probs) {
     const Problem &P = gr_internal_problem(h);
@@ -532,15 +653,27 @@ extern "C" int gr_batch_create(const gr_desc *descs, int n, gr_batch **out) {
     if (e == hipSuccess) e = hipEventCreate(&b->ev1);
     if (e == hipSuccess) e = hipGetDevice(&dev);
     if (e == hipSuccess) e = hipDeviceGetAttribute(&b->ncu, hipDeviceAttributeMultiprocessorCount, dev);
+    int coop = 0;
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev);
+    b->coop = coop != 0;
     for (int v = 0; v < kNumVariants && e == hipSuccess; ++v) {
       const BatchVariant &V = kVariants[v];
       e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
           &b->occupancy[v], reinterpret_cast<const void *>(b->bf16 ? V.bf16 : V.f32), 64 * V.bx * V.by, 0);
     }
-    for (int v = 0; v < kNumW2 && e == hipSuccess; ++v)
+    for (int v = 0; v < kNumVariants && e == hipSuccess; ++v) {
+      const BatchVariant &V = kVariants[v];
+      e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+          &b->run_occupancy[v], reinterpret_cast<const void *>(b->bf16 ? V.runbf16 : V.run32), 64 * V.bx * V.by, 0);
+    }
+    for (int v = 0; v < kNumW2 && e == hipSuccess; ++v) {
       e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b->w2_occupancy[v],
                                                        reinterpret_cast<const void *>(w2_kernel(b->bf16, v)),
                                                        kW2Lanes, 0);
+      if (e == hipSuccess)
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &b->w2_run_occupancy[v], reinterpret_cast<const void *>(w2_run_kernel(b->bf16, v)), kW2Lanes, 0);
+    }
     if (e == hipSuccess && b->w2_occupancy[0] < 1) b->w2_variant = 1;
     if (e != hipSuccess) rc = set_errorf(GR_EHIP, "gr_batch_create: %s", hipGetErrorString(e));
   }
@@ -574,17 +707,35 @@ extern "C" int gr_batch_set_config(gr_batch *b, int variant, int zchunk) {
   return GR_OK;
 }
 
+namespace {
+
+// mode 2 needs cooperative launches and, for the configured kernels, workgroups that fit a CU
+int check_mode(gr_batch *b, int mode, const char *who) {
+  if (mode < 0 || mode > 2) return set_errorf(GR_EINVAL, "%s: mode %d (0 launches, 1 graph, 2 one launch)", who, mode);
+  if (mode == 2 && !b->coop)
+    return set_errorf(GR_EUNSUPPORTED, "%s: the device does not support cooperative launches", who);
+  return GR_OK;
+}
+
+}  // namespace
+
 extern "C" int gr_batch_step(gr_batch *b, int napply, int use_graph) {
   if (!b) return set_error(GR_EINVAL, "gr_batch_step: null batch");
   if (napply < 0) return set_error(GR_EINVAL, "gr_batch_step: negative step count");
-  int rc = sync_problems(b);
+  int rc = check_mode(b, use_graph, "gr_batch_step");
+  if (rc) return rc;
+  if (use_graph == 2 && b->run_occupancy[b->variant] < 1)
+    return set_error(GR_EUNSUPPORTED, "gr_batch_step: the one-launch kernel of this tile variant does not fit a CU");
+  rc = sync_problems(b);
   if (rc) return rc;
   bool fuse = false;
   if (b->tblock == 2 && napply >= 2 && (rc = fused_ready(b, &fuse))) return rc;
   const int npasses = fuse ? napply / 2 : 0;
   if (fuse) rc = build_w2_tables(b);
   if (rc == GR_OK) rc = build_tables(b, false, npasses);
-  if (rc == GR_OK) rc = run_steps(b, 0, napply, use_graph != 0, fuse ? 2 : 1);
+  if (rc == GR_OK && fuse && use_graph == 2 && b->w2_run_occupancy[b->w2_variant] < 1)
+    rc = set_error(GR_EUNSUPPORTED, "gr_batch_step: the one-launch two-step kernel does not fit a CU");
+  if (rc == GR_OK) rc = run_steps(b, 0, napply, use_graph, fuse ? 2 : 1);
   if (rc != GR_OK) return rc;
   for (gr_handle *h : b->probs) {
     if (fuse)
@@ -612,6 +763,8 @@ extern "C" int gr_batch_set_time_block(gr_batch *b, int tblock, int block_varian
 extern "C" int gr_batch_autotune(gr_batch *b, int steps, int use_graph, int *variant, int *zchunk) {
   if (!b) return set_error(GR_EINVAL, "gr_batch_autotune: null batch");
   if (steps < 1) steps = 4;
+  if (int mrc = check_mode(b, use_graph, "gr_batch_autotune")) return mrc;
+  const int warm = use_graph == 2 ? 2 : 0;  // the warm-up loads the timed kernel's code object
   for (gr_handle *h : b->probs)  // allocate the spares up front: their set-up runs on the problems' streams
     if (!gr_internal_spare(h, 0)) return set_error(GR_ENOMEM, "gr_batch: cannot allocate tuning spares");
   int rc = sync_problems(b);
@@ -620,13 +773,13 @@ extern "C" int gr_batch_autotune(gr_batch *b, int steps, int use_graph, int *var
   double best = 1e30;  // ms per step
   int bv = b->variant, bz = b->zchunk;
   for (int v = 0; v < kNumVariants; ++v) {
-    if (b->occupancy[v] < 1) continue;
+    if (b->occupancy[v] < 1 || (use_graph == 2 && b->run_occupancy[v] < 1)) continue;
     for (int zc : zcands) {
       b->variant = v;
       b->zchunk = zc;
       if ((rc = build_tables(b, true))) return rc;
-      if ((rc = run_steps(b, 2, 1, false))) return rc;  // warm-up
-      if ((rc = run_steps(b, 2, steps, use_graph != 0))) return rc;
+      if ((rc = run_steps(b, 2, 1, warm))) return rc;  // warm-up
+      if ((rc = run_steps(b, 2, steps, use_graph))) return rc;
       if (b->last_ms / steps < best) {
         best = b->last_ms / steps;
         bv = v;
@@ -642,13 +795,13 @@ extern "C" int gr_batch_autotune(gr_batch *b, int steps, int use_graph, int *var
   if ((rc = fused_ready(b, &fuse))) return rc;
   const int passes = std::max(1, steps / 2);
   for (int wv = 0; wv < kNumW2 && fuse; ++wv) {
-    if (b->w2_occupancy[wv] < 1) continue;
+    if (b->w2_occupancy[wv] < 1 || (use_graph == 2 && b->w2_run_occupancy[wv] < 1)) continue;
     for (int zc : zcands) {
       b->w2_variant = wv;
       b->w2_zchunk = zc;
       if ((rc = build_w2_tables(b))) return rc;
-      if ((rc = run_steps(b, 0, 2, false, 2, true))) return rc;  // warm-up
-      if ((rc = run_steps(b, 0, 2 * passes, use_graph != 0, 2, true))) return rc;
+      if ((rc = run_steps(b, 0, 2, warm, 2, true))) return rc;  // warm-up
+      if ((rc = run_steps(b, 0, 2 * passes, use_graph, 2, true))) return rc;
       if (b->last_ms / (2 * passes) < best) {
         best = b->last_ms / (2 * passes);
         btb = 2;
@@ -674,7 +827,7 @@ extern "C" int gr_batch_get_stats(gr_batch *b, gr_batch_stats *out) {
   s.last_launches = b->last_launches;
   s.nproblems = (int)b->probs.size();
   s.items = b->nitems;
-  s.grid = b->grid;
+  s.grid = b->last_grid > 0 ? b->last_grid : b->grid;
   s.variant = b->variant;
   s.zchunk = b->zchunk;
   s.tile_x = 64 * kVariants[b->variant].bx;
@@ -682,9 +835,9 @@ extern "C" int gr_batch_get_stats(gr_batch *b, gr_batch_stats *out) {
   s.nvariants = kNumVariants;
   s.time_block = b->tblock;
   s.block_variant = b->w2_variant;
+  s.last_mode = b->last_mode;
   if (b->last_tblock == 2) {  // the last call ran two-step blocks: their items and tiles
     s.items = b->w2_items;
-    s.grid = b->w2_grid;
     s.zchunk = b->w2_zchunk;
     s.tile_x = W2Tile::TX;
     s.tile_y = W2Tile::TY;

@@ -63,7 +63,7 @@ class BatchStats(ctypes.Structure):
                 ("nproblems", ctypes.c_int32), ("items", ctypes.c_int32), ("grid", ctypes.c_int32),
                 ("variant", ctypes.c_int32), ("zchunk", ctypes.c_int32), ("tile_x", ctypes.c_int32),
                 ("tile_y", ctypes.c_int32), ("nvariants", ctypes.c_int32), ("time_block", ctypes.c_int32),
-                ("block_variant", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+                ("block_variant", ctypes.c_int32), ("last_mode", ctypes.c_int32)]
 
 
 EXPORTS = ["gr_create", "gr_destroy", "gr_last_error", "gr_upload", "gr_download", "gr_step", "gr_run_ts",
@@ -369,8 +369,11 @@ class Batch:
     def __getitem__(self, i):
         return self.problems[i]
 
-    def step(self, napply, use_graph=False):
-        check(lib().gr_batch_step(self._b, int(napply), 1 if use_graph else 0))
+    def step(self, napply, use_graph=False, mode=None):
+        """mode 0: a launch per step, 1: replayed from a graph (use_graph=True),
+        2: one cooperative launch for the whole call (gr_batch_step)."""
+        m = (1 if use_graph else 0) if mode is None else int(mode)
+        check(lib().gr_batch_step(self._b, int(napply), m))
 
     def set_config(self, variant=-1, zchunk=-1):
         check(lib().gr_batch_set_config(self._b, int(variant), int(zchunk)))
@@ -380,9 +383,10 @@ class Batch:
         uncapped) or 1 (capped at 128) (gr_batch_set_time_block)."""
         check(lib().gr_batch_set_time_block(self._b, int(tblock), int(block_variant)))
 
-    def autotune(self, steps=4, use_graph=False):
+    def autotune(self, steps=4, use_graph=False, mode=None):
         v, z = ctypes.c_int(0), ctypes.c_int(0)
-        check(lib().gr_batch_autotune(self._b, int(steps), 1 if use_graph else 0, ctypes.byref(v), ctypes.byref(z)))
+        m = (1 if use_graph else 0) if mode is None else int(mode)
+        check(lib().gr_batch_autotune(self._b, int(steps), m, ctypes.byref(v), ctypes.byref(z)))
         return v.value, z.value
 
     def stats(self):

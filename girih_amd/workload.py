@@ -18,7 +18,8 @@ from 8-layer velocity models, dims uniform in 192..512 step 16 from seed 2024, a
 Ricker source at each centre), with --config VARIANT,ZCHUNK[,TBLOCK[,BLOCK_VARIANT]]
 of gr_batch_set_config and gr_batch_set_time_block, or the batch autotuner; one
 persistent launch per step (TBLOCK 1) or per two-step block (2), replayed from a HIP
-graph.
+graph (--mode 1, the default), launched one by one (--mode 0), or all in one
+cooperative launch (--mode 2).
 """
 import argparse
 import json
@@ -45,13 +46,14 @@ def batch_main(a):
             b.set_config(cfg[0], cfg[1])
             b.set_time_block(cfg[2], cfg[3])
         else:
-            b.autotune(4, use_graph=True)
-        b.step(2, use_graph=True)
-        b.step(a.steps, use_graph=True)
+            b.autotune(4, mode=a.mode)
+        b.step(2, mode=a.mode)
+        b.step(a.steps, mode=a.mode)
         s = b.stats()
         print(json.dumps({"batch": len(sizes), "cells": cells,
                           "config": "%d,%d,%d,%d" % (s["variant"], s["zchunk"], s["time_block"], s["block_variant"]),
                           "tile": [s["tile_x"], s["tile_y"]], "steps": a.steps, "launches": s["last_launches"],
+                          "mode": s["last_mode"], "grid": s["grid"],
                           "glups": cells * a.steps / s["last_ms"] / 1e6}))
 
 
@@ -64,6 +66,8 @@ def main():
     ap.add_argument("--strm", type=int, choices=[-1, 0, 1], default=-1,
                     help="cache policy (gr_set_streaming): -1 by working-set size, 0 normal, 1 non-temporal")
     ap.add_argument("--graph", action="store_true", help="replay the timed steps from a HIP graph")
+    ap.add_argument("--mode", type=int, choices=[0, 1, 2], default=1,
+                    help="batch launches (gr_batch_step): 0 per step, 1 graph replay, 2 one cooperative launch")
     ap.add_argument("--dtype", choices=["f32", "f64"], default="f32")
     ap.add_argument("--bf16", action="store_true", help="coefficient arrays / roc2 stored as bf16")
     ap.add_argument("--config", default=None)

@@ -234,7 +234,7 @@ typedef struct gr_batch_stats {
   int32_t time_block;       /* configured steps per launch (1 or 2); items, grid, zchunk and
                                tile_x/y describe the two-step blocks when the last call ran them */
   int32_t block_variant;    /* two-step kernel (gr_batch_set_time_block) */
-  int32_t reserved;
+  int32_t last_mode;        /* mode of the last gr_batch_step / tuning run (0, 1 or 2) */
 } gr_batch_stats;
 int gr_batch_create(const gr_desc *descs, int n, gr_batch **out);
 int gr_batch_destroy(gr_batch *b);
@@ -245,8 +245,16 @@ int gr_batch_problem(gr_batch *b, int i, gr_handle **h);
  * unchanged.  Results never depend on them. */
 int gr_batch_set_config(gr_batch *b, int variant, int zchunk);
 /* Advance every problem by napply applications (girih's U1/U2 pairing per
- * problem).  use_graph != 0 replays the step loop from a HIP graph (captured on
- * first use, kept while napply and the configuration stay the same). */
+ * problem).  use_graph selects how the steps are launched:
+ *   0  one persistent launch per step (or two-step block);
+ *   1  the same launches replayed from a HIP graph (captured on first use, kept
+ *      while napply and the configuration stay the same);
+ *   2  one cooperative persistent launch for the whole call: its workgroups run a
+ *      step's work queue, then meet at a grid barrier whose agent-scope release
+ *      and acquire make the step's writes visible to every XCD.  With two-step
+ *      blocks an odd count adds one single-step launch.  GR_EUNSUPPORTED where
+ *      the device has no cooperative launch.
+ * Results do not depend on the mode. */
 int gr_batch_step(gr_batch *b, int napply, int use_graph);
 /* Time steps per launch (no counterpart in girih, whose diamonds block time per
  * grid, src/kernels/diamond_ts.c): 1 = single steps; 2 = two-step blocks, where a
@@ -260,8 +268,9 @@ int gr_batch_step(gr_batch *b, int napply, int use_graph);
  * Results never depend on either. */
 int gr_batch_set_time_block(gr_batch *b, int tblock, int block_variant);
 /* Time every tile variant x z chunk for `steps` steps, then two-step blocks per z
- * chunk, on scratch fields, and keep the fastest per step (*zchunk reports the
- * chosen kind's chunk); the problems' fields and parities are untouched. */
+ * chunk, on scratch fields, launched as use_graph says (gr_batch_step), and keep
+ * the fastest per step (*zchunk reports the chosen kind's chunk); the problems'
+ * fields and parities are untouched. */
 int gr_batch_autotune(gr_batch *b, int steps, int use_graph, int *variant, int *zchunk);
 int gr_batch_get_stats(gr_batch *b, gr_batch_stats *out);
 

@@ -9,6 +9,8 @@
   rotated by their own fused passes; a problem whose U1 and U2 halos differ makes
   the call fall back to single steps;
 * graph replays of the step loop equal eager launches on every replay;
+* one cooperative launch per call (mode 2, grid barrier between steps) equals a
+  launch per step, for single steps and two-step blocks;
 * against the FP64 CPU oracle, with the roc2 the batch actually holds, the FP32
   batch stays within a relative L2 error of 1e-5;
 * the layered velocity model has the stated layer speeds and bf16 rounding.
@@ -173,6 +175,60 @@ def test_batch_two_step_falls_back_on_unequal_halos():
         np.testing.assert_array_equal(a2, b2)
 
 
+@pytest.mark.parametrize("bf16", [True, False])
+@pytest.mark.parametrize("tblock", [1, 2])
+def test_batch_one_launch_equals_launch_per_step(bf16, tblock):
+    """Mode 2 runs a call's steps (or two-step blocks) in one cooperative launch;
+    with problems at mixed parities and a source, it gives the bits of a launch per
+    step, call after call, and each problem's bits stepped alone."""
+    flags = gpu.GR_FAST | (gpu.GR_COEF_BF16 if bf16 else 0)
+    src = np.linspace(0.3, -0.4, 9)
+    print("one launch", bf16, tblock, flush=True)
+    with gpu.Batch(SIZES, flags=flags, coef=physics.fd8_coef()) as per, \
+            gpu.Batch(SIZES, flags=flags, coef=physics.fd8_coef()) as one:
+        for b in (per, one):
+            _init(b, 23, src)
+            b.problems[1].step(1)
+            b.set_config(1, 8)
+            b.set_time_block(tblock, 1 if tblock == 2 else -1)
+        for n in (5, 4, 3):
+            per.step(n)
+            one.step(n, mode=2)
+            st = one.stats()
+            assert st["last_mode"] == 2
+            expect = 1 if tblock == 1 else 1 + (n & 1)
+            assert st["last_launches"] == expect, (n, st)
+            for i in range(len(SIZES)):
+                P = per.problems[i].download()
+                O = one.problems[i].download()
+                np.testing.assert_array_equal(P[0], O[0], err_msg="problem %d U1 after %d" % (i, n))
+                np.testing.assert_array_equal(P[1], O[1], err_msg="problem %d U2 after %d" % (i, n))
+        got = [p.download() for p in one.problems]
+    for i in (0, 1):
+        U1, U2 = _alone(SIZES[i], flags, 23, i, 12, src, pre=1 if i == 1 else 0)
+        np.testing.assert_array_equal(got[i][0], U1)
+        np.testing.assert_array_equal(got[i][1], U2)
+
+
+def test_batch_one_launch_autotune_keeps_fields():
+    flags = gpu.GR_FAST | gpu.GR_COEF_BF16
+    with gpu.Batch(SIZES[:3], flags=flags, coef=physics.fd8_coef()) as b:
+        _init(b, 29)
+        before = [p.download() for p in b.problems]
+        b.autotune(2, mode=2)
+        assert b.stats()["last_mode"] == 2
+        after = [p.download() for p in b.problems]
+        for x, y in zip(before, after):
+            np.testing.assert_array_equal(x[0], y[0])
+            np.testing.assert_array_equal(x[1], y[1])
+        b.step(3, mode=2)
+        got = [p.download() for p in b.problems]
+    for i, size in enumerate(SIZES[:3]):
+        U1, U2 = _alone(size, flags, 29, i, 3)
+        np.testing.assert_array_equal(got[i][0], U1)
+        np.testing.assert_array_equal(got[i][1], U2)
+
+
 def test_batch_time_block_errors():
     with gpu.Batch(SIZES[:2], flags=gpu.GR_FAST, coef=physics.fd8_coef()) as b:
         for bad in (0, 3, 4):
@@ -182,6 +238,9 @@ def test_batch_time_block_errors():
             b.set_time_block(2, 2)
         b.set_time_block(1)
         assert b.stats()["time_block"] == 1
+        for bad in (-1, 3):
+            with pytest.raises(gpu.GirihError):
+                b.step(1, mode=bad)
 
 
 def test_batch_autotune_keeps_fields():
