@@ -12,6 +12,7 @@ struct Problem {
   int kernel = 0, dtype = 0, flags = 0;
   int nx = 0, ny = 0, nz = 0, R = 0, ncoef = 0;
   int xoff = 0, pitch = 0, rows = 0, planes = 0;
+  int cxoff = 0;  // x offset of the roc2 / coefficient storage (xoff, or its own for bf16 storage)
   long long field_elems = 0;  // elements of one field allocation
   double c[5] = {0, 0, 0, 0, 0};
   double roc2s = 0;

@@ -259,7 +259,13 @@ extern "C" int gr_create(const gr_desc *d, gr_handle **out) {
   const int es = (int)elem_size(P.dtype);
   const int line = 128 / es;                      // elements per 128-byte line
   P.xoff = (line - P.R % line) % line;            // (xoff + R) % line == 0
-  P.pitch = ((P.xoff + P.nx + 2 * P.R + line - 1) / line) * line;
+  // bf16 roc2 / coefficient storage: rows of a whole number of 128-byte lines (pitch a
+  // multiple of 64 elements) and its own x offset, so that its first interior cell
+  // (i = R) starts a line too; its element index equals the fields'
+  const bool bf16_store = (P.flags & GR_COEF_BF16) != 0;
+  const int pline = bf16_store ? 64 : line;
+  P.pitch = ((P.xoff + P.nx + 2 * P.R + pline - 1) / pline) * pline;
+  P.cxoff = bf16_store ? (64 - P.R % 64) % 64 : P.xoff;
   P.rows = P.ny + 2 * P.R;
   P.planes = P.nz + 2 * P.R;
   P.field_elems = (long long)P.pitch * P.rows * P.planes;
@@ -301,9 +307,11 @@ extern "C" int gr_create(const gr_desc *d, gr_handle **out) {
   };
   alloc(&h->fbuf[0], fbytes);
   alloc(&h->fbuf[1], fbytes);
-  const size_t cbytes = (P.flags & GR_COEF_BF16) ? (size_t)P.field_elems * 2 : fbytes;
-  if (P.kernel == 0 && !(P.flags & GR_ROC2_SCALAR)) alloc(&h->alloc_roc2, cbytes);
-  if (P.ncoef > 0) alloc(&h->alloc_coef, cbytes * P.ncoef);
+  // bf16 storage: 64 elements of slack for its x offset (up to 63 past the fields')
+  const size_t cbytes = bf16_store ? (size_t)P.field_elems * 2 : fbytes;
+  const size_t cslack = bf16_store ? 128 : 0;
+  if (P.kernel == 0 && !(P.flags & GR_ROC2_SCALAR)) alloc(&h->alloc_roc2, cbytes + cslack);
+  if (P.ncoef > 0) alloc(&h->alloc_coef, cbytes * P.ncoef + cslack);
   // the fields are zero before any other stream (a batch, torch) can touch them
   if (rc == GR_OK && (e = hipStreamSynchronize(h->stream)) != hipSuccess)
     rc = fail(GR_EHIP, "hipStreamSynchronize: %s", hipGetErrorString(e));
@@ -312,9 +320,10 @@ extern "C" int gr_create(const gr_desc *d, gr_handle **out) {
     return rc;
   }
   const size_t ces = (P.flags & GR_COEF_BF16) ? 2 : es;
-  P.roc2 = h->alloc_roc2 ? (char *)h->alloc_roc2 + (size_t)P.xoff * ces : nullptr;
-  P.coef = h->alloc_coef ? (char *)h->alloc_coef + (size_t)P.xoff * ces : nullptr;
-  h->stats.device_bytes = (int64_t)(2 * fbytes + (h->alloc_roc2 ? cbytes : 0) + (size_t)P.ncoef * cbytes);
+  P.roc2 = h->alloc_roc2 ? (char *)h->alloc_roc2 + (size_t)P.cxoff * ces : nullptr;
+  P.coef = h->alloc_coef ? (char *)h->alloc_coef + (size_t)P.cxoff * ces : nullptr;
+  h->stats.device_bytes = (int64_t)(2 * fbytes + (h->alloc_roc2 ? cbytes + cslack : 0) +
+                                    (h->alloc_coef ? (size_t)P.ncoef * cbytes + cslack : 0));
   *out = h;
   return GR_OK;
 }
@@ -830,13 +839,17 @@ extern "C" int gr_autotune(gr_handle *h, int steps, gr_stats *best) {
   return GR_OK;
 }
 
+static size_t coef_elem_size(const Problem &P) { return (P.flags & GR_COEF_BF16) ? 2 : elem_size(P.dtype); }
+
 extern "C" int gr_field_ptr(gr_handle *h, int which, void **p) {
   if (!h || !p) return fail(GR_EINVAL, "gr_field_ptr: null argument");
   switch (which) {
     case 0: *p = h->fbuf[h->lu[0]]; break;
     case 1: *p = h->fbuf[h->lu[1]]; break;
-    case 2: *p = h->alloc_roc2; break;
-    case 3: *p = h->alloc_coef; break;
+    // roc2 / coefficients: the pointer for which base + x_offset + index holds element index
+    // (bf16 storage sits at its own offset inside its allocation)
+    case 2: *p = h->P.roc2 ? (char *)h->P.roc2 - (size_t)h->P.xoff * coef_elem_size(h->P) : nullptr; break;
+    case 3: *p = h->P.coef ? (char *)h->P.coef - (size_t)h->P.xoff * coef_elem_size(h->P) : nullptr; break;
     case 4:
     case 5: {
       int sp[2];
