@@ -6,7 +6,8 @@
 
 The fields start from the seeded uniform state (--init random: interior U(0,1);
 coefficient arrays centre U[0.3,0.5), neighbours U[0,0.1); kernel 0: the 1024^3
-wave set-up, v = 3000 m/s, h = 10 m, CFL 0.4, scalar roc2), from zero fields
+wave set-up, v = 3000 m/s, h = 10 m, CFL 0.4, scalar roc2; with --roc2-field or
+--bf16, the batch's 8-layer velocity model as a per-cell roc2), from zero fields
 (--init zero) or from girih's closed-form state (--init girih).  --config is
 VARIANT,ZCHUNK,XCD_ORDER,T_BLOCK as in gr_set_config / gr_set_xcd_order; with no
 --config the autotuner picks one.  One line of JSON: the configuration, the
@@ -69,7 +70,9 @@ def main():
     ap.add_argument("--mode", type=int, choices=[0, 1, 2], default=1,
                     help="batch launches (gr_batch_step): 0 per step, 1 graph replay, 2 one cooperative launch")
     ap.add_argument("--dtype", choices=["f32", "f64"], default="f32")
-    ap.add_argument("--bf16", action="store_true", help="coefficient arrays / roc2 stored as bf16")
+    ap.add_argument("--bf16", action="store_true",
+                    help="coefficient arrays / roc2 stored as bf16 (kernel 0: a bf16 roc2 field)")
+    ap.add_argument("--roc2-field", action="store_true", help="kernel 0: a per-cell roc2 field (layered model)")
     ap.add_argument("--config", default=None)
     ap.add_argument("--steps", type=int, default=12)
     ap.add_argument("--init", choices=["random", "zero", "girih"], default="random")
@@ -84,8 +87,9 @@ def main():
     dtype = gpu.GR_F32 if a.dtype == "f32" else gpu.GR_F64
     flags = gpu.GR_COEF_BF16 if a.bf16 else 0
     coef = None
+    roc2_field = a.kernel == 0 and (a.bf16 or a.roc2_field)
     if a.kernel == 0:
-        flags |= gpu.GR_ROC2_SCALAR | (gpu.GR_FAST if dtype == gpu.GR_F32 else 0)
+        flags |= (0 if roc2_field else gpu.GR_ROC2_SCALAR) | (gpu.GR_FAST if dtype == gpu.GR_F32 else 0)
         coef = physics.fd8_coef()
     elif a.kernel == 1:
         coef = [0.4, 0.1]
@@ -93,6 +97,8 @@ def main():
     st.set_streaming(a.strm)
     if gpu.NUM_COEF_ARRAYS[a.kernel]:
         st.init_coef_uniform(5, 0.3, 0.5, 0.0, 0.1)
+    if roc2_field:  # the batch's 8-layer velocity model (h = 10 m, CFL 0.4 at 4725 m/s)
+        st.init_roc2_model(gpu.vmodel(2024, physics.cfl_dt(4500.0 * 1.05, 10.0, 0.4) / 10.0))
     st.init_fields({"girih": 0, "random": 1, "zero": 2}[a.init], 12345)
     if a.config:
         var, zc, order, tb = (int(x) for x in a.config.split(","))
