@@ -428,21 +428,11 @@ static void serial_sweep(int kernel, int R, int nx_, int ny, int nz_, const real
 #undef C
 }
 
-static void verify(Params *p) {
-  printf("#ts:%s stencil:%s|R:%d|T:%d|%s nt:%03d thrd:%d prec.:%s %s dom:(%d,%03d,%03d) top:(1,1,1) ",
-         TS_NAME[p->target_ts], KINFO[p->target_kernel].name, p->R, KINFO[p->target_kernel].time_order,
-         KINFO[p->target_kernel].coeff, p->nt, 1, DP ? "DP" : "SP", "no-concat", p->nx, p->ny, p->nz);
-  if (p->target_ts == 2) printf("TB:%d wf:%d ", p->t_dim, 1);
-  fflush(stdout);
-  arrays_allocate(p);
-  init_coeff(p);
-  domain_data_fill(p);
-  gr_handle *h = gpu_setup(p);
-  gpu_ts(p, h);
-  check(gr_download(h, p->ld, p->U1, p->U2), "gr_download");
-  gr_destroy(h);
-
-  /* serial run on unpadded copies of the same initial state */
+/* The serial reference of --verify: the initial state rebuilt from scratch
+ * (domain_data_fill) on unpadded copies (x extent nx+2R, as verification.c:52-312
+ * allocates), with p's coefficient arrays, stepped serially for the run's
+ * application count.  *pu / *pv receive U1 / U2 (caller frees both). */
+static void serial_reference(const Params *p, real_t **pu, real_t **pv) {
   const int R = p->R, ux = p->nx + 2 * R, uy = p->ld[1], uz = p->ld[2];
   const uint64_t un = (uint64_t)ux * uy * uz;
   const int nc = KINFO[p->target_kernel].ncoef;
@@ -454,6 +444,7 @@ static void verify(Params *p) {
   q.U1 = calloc(p->ln_domain, sizeof(real_t));
   q.U2 = calloc(p->ln_domain, sizeof(real_t));
   q.U3 = p->U3 ? calloc(p->ln_domain, sizeof(real_t)) : NULL;
+  if (!q.U1 || !q.U2 || (p->U3 && !q.U3)) die("no sufficient memory");
   q.verbose = 0;
   domain_data_fill(&q);
   for (uint64_t row = 0; row < (uint64_t)uy * uz; row++) {
@@ -471,6 +462,32 @@ static void verify(Params *p) {
     else
       serial_sweep(p->target_kernel, R, ux, uy, uz, cf, v, u, r2);
   }
+  free(r2);
+  if (nc) free(cf);
+  free(q.U1);
+  free(q.U2);
+  free(q.U3);
+  *pu = u;
+  *pv = v;
+}
+
+static void verify(Params *p) {
+  printf("#ts:%s stencil:%s|R:%d|T:%d|%s nt:%03d thrd:%d prec.:%s %s dom:(%d,%03d,%03d) top:(1,1,1) ",
+         TS_NAME[p->target_ts], KINFO[p->target_kernel].name, p->R, KINFO[p->target_kernel].time_order,
+         KINFO[p->target_kernel].coeff, p->nt, 1, DP ? "DP" : "SP", "no-concat", p->nx, p->ny, p->nz);
+  if (p->target_ts == 2) printf("TB:%d wf:%d ", p->t_dim, 1);
+  fflush(stdout);
+  arrays_allocate(p);
+  init_coeff(p);
+  domain_data_fill(p);
+  gr_handle *h = gpu_setup(p);
+  gpu_ts(p, h);
+  check(gr_download(h, p->ld, p->U1, p->U2), "gr_download");
+  gr_destroy(h);
+
+  real_t *u = NULL, *v = NULL;
+  serial_reference(p, &u, &v);
+  const int R = p->R, ux = p->nx + 2 * R, uy = p->ld[1];
   /* compare_results_std (verification.c:823-860) on U1 */
   real_t diff_l1 = 0, max_error = 0, abs_diff;
   for (int k = 0; k < p->nz; k++)
@@ -491,11 +508,6 @@ static void verify(Params *p) {
   printf("-PASSED\n");
   free(u);
   free(v);
-  free(r2);
-  if (nc) free(cf);
-  free(q.U1);
-  free(q.U2);
-  free(q.U3);
   arrays_free(p);
 }
 
