@@ -20,7 +20,8 @@
 // The forward pass stores (x_{cK}, x_{cK+1}) every K applications; the backward
 // pass recomputes each segment's K states from its checkpoint into a buffer and
 // walks it backwards.  Memory: 2*ceil(nt/K) + 2 fields for the forward pass;
-// the first backward pass adds K + 4 fields and a dense accumulator.
+// the first backward pass adds 2K + 4 fields (the segment's states and adjoint
+// fields) and a dense accumulator, which takes a segment's gradient terms in one pass.
 #include <hip/hip_runtime.h>
 #include <hip/hip_bf16.h>
 
@@ -59,21 +60,36 @@ __global__ void inject_kernel_same(T *nu, const T *roc2, long long sy, long long
   nu[o] = nu[o] + roc2[o] * res[(long long)j * nx + i];
 }
 
-// g += nu * ((x2 - x1) - (x1 - x0) - s e_src) over the interior (g dense [nz][ny][nx]).
-// The differences are formed in FP64 (exact for FP32 states); g has the field type:
-// an FP32 shot's gradient error is set by its FP32 traces (~1e-6 relative), far
-// above FP32 accumulation over the steps.
+// g += nu_{a+2} * ((x_{a+2} - x_{a+1}) - (x_{a+1} - x_a) - s_a e_src) over the
+// interior (g dense [nz][ny][nx]) for the steps a = a1-1 down to a0 of one checkpoint
+// segment, in one pass: per cell the terms are added in the order of the backward
+// walk, as the CPU adjoint adds them, but g is read and written once per segment and
+// each state and adjoint field once.  The differences are formed in FP64 (exact for
+// FP32 states); g has the field type: an FP32 shot's gradient error is set by its FP32
+// traces, far above FP32 accumulation over the steps.  States x_{a0} .. x_{a1+1} are
+// fields xs + t * fstride; nu_{a+2} is field nus + (a - a0) * fstride.  The source
+// term applies at cell src_idx for steps a < nsrc.
 template <typename T>
-__global__ void grad_accum_kernel(T *g, const T *nu, const T *x2, const T *x1, const T *x0, long long sy, long long sz,
-                                  int nx, int ny, int R, const T *src, long long src_idx) {
+__global__ void grad_segment_kernel(T *g, const T *__restrict__ nus, const T *__restrict__ xs, long long fstride,
+                                    long long sy, long long sz, int nx, int ny, int R, int nsteps, const T *src,
+                                    long long src_idx, int a0, int nsrc) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y, k = blockIdx.z;
   if (i >= nx) return;
   const long long o = (long long)(k + R) * sz + (long long)(j + R) * sy + (i + R);
-  const double a = (double)x2[o], b = (double)x1[o], c = (double)x0[o];
-  double d = (a - b) - (b - c);
-  if (src && o == src_idx) d = d - (double)*src;
   T &gc = g[((long long)k * ny + j) * nx + i];
-  gc = (T)((double)gc + (double)nu[o] * d);
+  T acc = gc;
+  const bool at_src = src && o == src_idx;
+  double x2 = (double)xs[(long long)(nsteps + 1) * fstride + o];  // x_{a+2} for a = a1 - 1
+  double x1 = (double)xs[(long long)nsteps * fstride + o];
+  for (int t = nsteps - 1; t >= 0; --t) {  // t = a - a0
+    const double x0 = (double)xs[(long long)t * fstride + o];
+    double d = (x2 - x1) - (x1 - x0);
+    if (at_src && a0 + t < nsrc) d = d - (double)src[a0 + t];
+    acc = (T)((double)acc + (double)nus[(long long)t * fstride + o] * d);
+    x2 = x1;
+    x1 = x0;
+  }
+  gc = acc;
 }
 
 template <typename T>
@@ -127,8 +143,8 @@ struct gr_shot {
   double dt_over_h = 0;
   std::vector<void *> allocs;  // everything below lives in these
   std::vector<void *> ck;      // 2*nck checkpoint fields (x_{cK}, x_{cK+1})
-  std::vector<void *> seg;     // ckpt+2 recompute fields
-  void *nu[2] = {nullptr, nullptr};
+  void *seg = nullptr;         // ckpt+2 recompute fields x_{a0} .. x_{a1+1}, contiguous
+  void *nus = nullptr;         // ckpt+2 adjoint fields nu_{a0+2} .. nu_{a1+3}, contiguous
   void *x[2] = {nullptr, nullptr};  // forward ping-pong
   void *g = nullptr;  // gradient accumulator, field type, dense interior
   bool have_forward = false;
@@ -169,20 +185,26 @@ hipError_t inject_any(const Problem &P, void *nu, int rec_k, const void *res, hi
   return hipGetLastError();
 }
 
-hipError_t grad_any(const Problem &P, void *g, const void *nu, const void *x2, const void *x1, const void *x0,
-                    const void *src, long long sidx, hipStream_t st) {
+// the accumulation of one segment's nsteps steps starting at a0 (grad_segment_kernel)
+hipError_t grad_segment(gr_handle *h, void *g, const void *nus, const void *xs, int nsteps, int a0, hipStream_t st) {
+  const Problem &P = gr_internal_problem(h);
   const long long sz = (long long)P.pitch * P.rows;
   const dim3 grid((P.nx + 255) / 256, P.ny, P.nz);
+  int nsrc = 0;
+  long long sidx = -1;
+  const void *src = gr_internal_source(h, &nsrc, &sidx);
+  if (!src || sidx < 0) {
+    src = nullptr;
+    sidx = -1;
+  }
   if (P.dtype == GR_F64)
-    hipLaunchKernelGGL(grad_accum_kernel<double>, grid, dim3(256), 0, st, (double *)g, (const double *)nu,
-                       (const double *)x2,
-                       (const double *)x1, (const double *)x0, (long long)P.pitch, sz, P.nx, P.ny, P.R,
-                       (const double *)src, sidx);
+    hipLaunchKernelGGL(grad_segment_kernel<double>, grid, dim3(256), 0, st, (double *)g, (const double *)nus,
+                       (const double *)xs, (long long)P.field_elems, (long long)P.pitch, sz, P.nx, P.ny, P.R, nsteps,
+                       (const double *)src, sidx, a0, nsrc);
   else
-    hipLaunchKernelGGL(grad_accum_kernel<float>, grid, dim3(256), 0, st, (float *)g, (const float *)nu,
-                       (const float *)x2,
-                       (const float *)x1, (const float *)x0, (long long)P.pitch, sz, P.nx, P.ny, P.R,
-                       (const float *)src, sidx);
+    hipLaunchKernelGGL(grad_segment_kernel<float>, grid, dim3(256), 0, st, (float *)g, (const float *)nus,
+                       (const float *)xs, (long long)P.field_elems, (long long)P.pitch, sz, P.nx, P.ny, P.R, nsteps,
+                       (const float *)src, sidx, a0, nsrc);
   return hipGetLastError();
 }
 
@@ -349,19 +371,17 @@ extern "C" int gr_shot_adjoint(gr_shot *s, const void *res_dev, void *grad_dev) 
   const size_t plane = (size_t)P.nx * P.ny * gr::elem_size(P.dtype);
   const long long ncell = (long long)P.nx * P.ny * P.nz;
   hipError_t e = hipSuccess;
-  if (s->seg.empty()) {  // backward-pass buffers, allocated on first use (forward-only shots never hold them)
+  const int nslot = s->ckpt + 2;
+  if (!s->seg) {  // backward-pass buffers, allocated on first use (forward-only shots never hold them)
     const size_t before = s->allocs.size();
-    auto field = [&](void **p) -> hipError_t {
-      hipError_t r = hipMalloc(p, fbytes);
+    auto fields = [&](void **p, int n) -> hipError_t {
+      hipError_t r = hipMalloc(p, fbytes * (size_t)n);
       if (r == hipSuccess) s->allocs.push_back(*p);
-      if (r == hipSuccess) r = hipMemsetAsync(*p, 0, fbytes, st);  // zero halo, ordered before first use
+      if (r == hipSuccess) r = hipMemsetAsync(*p, 0, fbytes * (size_t)n, st);  // zero halo, ordered before first use
       return r;
     };
-    for (int i = 0; i < s->ckpt + 2 && e == hipSuccess; ++i) {
-      void *p = nullptr;
-      if ((e = field(&p)) == hipSuccess) s->seg.push_back(p);
-    }
-    for (int i = 0; i < 2 && e == hipSuccess; ++i) e = field(&s->nu[i]);
+    e = fields(&s->seg, nslot);
+    if (e == hipSuccess) e = fields(&s->nus, nslot);
     if (e == hipSuccess) {
       e = hipMalloc(&s->g, gr::elem_size(P.dtype) * (size_t)ncell);
       if (e == hipSuccess) s->allocs.push_back(s->g);
@@ -369,13 +389,13 @@ extern "C" int gr_shot_adjoint(gr_shot *s, const void *res_dev, void *grad_dev) 
     if (e != hipSuccess) {  // release the partial set so a later call starts clean
       for (size_t i = before; i < s->allocs.size(); ++i) (void)hipFree(s->allocs[i]);
       s->allocs.resize(before);
-      s->seg.clear();
-      s->nu[0] = s->nu[1] = nullptr;
+      s->seg = s->nus = nullptr;
       s->g = nullptr;
       return gr::set_errorf(GR_ENOMEM, "gr_shot_adjoint: device allocation failed: %s", hipGetErrorString(e));
     }
   }
-  for (int i = 0; i < 2 && e == hipSuccess; ++i) e = hipMemsetAsync(s->nu[i], 0, fbytes, st);
+  auto segf = [&](int t) { return (void *)((char *)s->seg + (size_t)t * fbytes); };  // x_{a0+t}
+  auto nuf = [&](int t) { return (void *)((char *)s->nus + (size_t)t * fbytes); };   // nu_{a0+2+t}
   if (e == hipSuccess) {
     if (P.dtype == GR_F64)
       hipLaunchKernelGGL(zero_kernel<double>, dim3((unsigned)((ncell + 255) / 256)), dim3(256), 0, st, (double *)s->g,
@@ -385,30 +405,34 @@ extern "C" int gr_shot_adjoint(gr_shot *s, const void *res_dev, void *grad_dev) 
                          ncell);
     e = hipGetLastError();
   }
-  void *n0 = s->nu[0], *n1 = s->nu[1];  // nu_{a+4}, nu_{a+3}
   for (int c = s->nck - 1; c >= 0 && e == hipSuccess; --c) {
-    const int a0 = c * s->ckpt, a1 = std::min(a0 + s->ckpt, s->nt);
-    // recompute x_{a0} .. x_{a1+1} into seg[0 .. a1-a0+1]
-    e = hipMemcpyAsync(s->seg[0], s->ck[2 * c], fbytes, hipMemcpyDeviceToDevice, st);
-    if (e == hipSuccess) e = hipMemcpyAsync(s->seg[1], s->ck[2 * c + 1], fbytes, hipMemcpyDeviceToDevice, st);
+    const int a0 = c * s->ckpt, a1 = std::min(a0 + s->ckpt, s->nt), K = a1 - a0;
+    // recompute x_{a0} .. x_{a1+1} into seg slots 0 .. K+1
+    e = hipMemcpyAsync(segf(0), s->ck[2 * c], fbytes, hipMemcpyDeviceToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(segf(1), s->ck[2 * c + 1], fbytes, hipMemcpyDeviceToDevice, st);
     for (int a = a0; a < a1 && e == hipSuccess; ++a) {
       const int t = a - a0 + 2;
       long long sidx = -1;
       const void *sv = src_of(h, a, &sidx);
-      e = gr::sweep_fields(P, fbase(P, s->seg[t]), fbase(P, s->seg[t - 1]), sv, sidx, lc, st,
-                           fbase(P, s->seg[t - 2]));
+      e = gr::sweep_fields(P, fbase(P, segf(t)), fbase(P, segf(t - 1)), sv, sidx, lc, st, fbase(P, segf(t - 2)));
     }
-    for (int a = a1 - 1; a >= a0 && e == hipSuccess; --a) {
-      const int t = a - a0 + 2;
-      e = gr::sweep_fields(P, fbase(P, n0), fbase(P, n1), nullptr, -1, lc, st);  // n0 <- nu_{a+2}
-      if (e == hipSuccess) e = inject_any(P, fbase(P, n0), s->rec_k, (const char *)res_dev + (size_t)a * plane, st);
-      long long sidx = -1;
-      const void *sv = src_of(h, a, &sidx);
-      if (e == hipSuccess)
-        e = grad_any(P, s->g, fbase(P, n0), fbase(P, s->seg[t]), fbase(P, s->seg[t - 1]), fbase(P, s->seg[t - 2]),
-                     sv, sidx, st);
-      std::swap(n0, n1);
+    // nu_{a1+2}, nu_{a1+3} into slots K, K+1: zero after the last step, else the two
+    // lowest of the later segment (its slots 0 and 1)
+    if (e == hipSuccess) {
+      if (c == s->nck - 1) {
+        e = hipMemsetAsync(nuf(K), 0, fbytes, st);
+        if (e == hipSuccess) e = hipMemsetAsync(nuf(K + 1), 0, fbytes, st);
+      } else {  // slot 1 first: with K = 1 slot 1 is also a destination
+        e = hipMemcpyAsync(nuf(K + 1), nuf(1), fbytes, hipMemcpyDeviceToDevice, st);
+        if (e == hipSuccess) e = hipMemcpyAsync(nuf(K), nuf(0), fbytes, hipMemcpyDeviceToDevice, st);
+      }
     }
+    for (int a = a1 - 1; a >= a0 && e == hipSuccess; --a) {  // nu_{a+2} <- f(nu_{a+3}, nu_{a+4})
+      const int t = a - a0;
+      e = gr::sweep_fields(P, fbase(P, nuf(t)), fbase(P, nuf(t + 1)), nullptr, -1, lc, st, fbase(P, nuf(t + 2)));
+      if (e == hipSuccess) e = inject_any(P, fbase(P, nuf(t)), s->rec_k, (const char *)res_dev + (size_t)a * plane, st);
+    }
+    if (e == hipSuccess) e = grad_segment(h, s->g, fbase(P, nuf(0)), fbase(P, segf(0)), K, a0, st);
   }
   if (e == hipSuccess) {
     const double scale = -1.0 / (s->dt_over_h * s->dt_over_h);
