@@ -159,21 +159,48 @@ __global__ __launch_bounds__(64 * BX * BY) void batch_run_kernel(const BatchDesc
 // chip by the two-step pass's tile body (gr_wave2.hpp): it reads (u^(n-1), u^n) and
 // writes (u^(n+1), u^(n+2)) into the problem's spare pair.  `pass` counts the blocks
 // of the call; the block adds source values 2*pass and 2*pass + 1 of the call.
-template <typename C>
 struct W2BatchDesc {
-  W2Args<float, C> a;  // buffers of this block, tiles, z chunk, coefficients, source
-  int item0;           // first work item of this problem
-  int nsrc;            // source values available from a.src
+  W2Args<float, float> a;  // buffers of this block, tiles, z chunk, coefficients, source; a.roc2
+                           // addresses the roc2 field, whose storage type the kernel instantiation names
+  int item0;               // first work item of this problem
+  int nsrc;                // source values available from a.src
 };
-static_assert(sizeof(W2BatchDesc<float>) % 8 == 0, "descriptor words");
-static_assert(sizeof(W2BatchDesc<float>) == sizeof(W2BatchDesc<__hip_bfloat16>), "one table size for both");
+static_assert(sizeof(W2BatchDesc) % 8 == 0, "descriptor words");
+
+// The tile body's arguments with the roc2 field read as C (FP32 or bf16 storage).
+template <typename C>
+__device__ __forceinline__ W2Args<float, C> w2_args_as(const W2Args<float, float> &x) {
+  W2Args<float, C> a;
+  a.P = x.P;
+  a.V = x.V;
+  a.W = x.W;
+  a.Y = x.Y;
+  a.roc2 = reinterpret_cast<const C *>(x.roc2);
+  for (int r = 0; r < 5; ++r) a.c[r] = x.c[r];
+  a.csum = x.csum;
+  a.roc2s = x.roc2s;
+  a.sz = x.sz;
+  a.sy = x.sy;
+  a.nx = x.nx;
+  a.ny = x.ny;
+  a.nz = x.nz;
+  a.tiles_x = x.tiles_x;
+  a.tiles_y = x.tiles_y;
+  a.nzc = x.nzc;
+  a.zchunk = x.zchunk;
+  a.xcd_order = x.xcd_order;
+  a.src = x.src;
+  a.src_k = x.src_k;
+  a.src_off = x.src_off;
+  return a;
+}
 
 using W2Tile = Wave2Tile<float, float, true, false, true, 1, 8, 2>;  // 64 x 16 output tiles, 512 lanes
 constexpr int kW2Lanes = W2Tile::NT;
 
 // Claim and run the items of one two-step block until the queue is empty.
 template <typename C>
-__device__ __forceinline__ void block_items(const W2BatchDesc<C> *__restrict__ d, int nprob, int nitems,
+__device__ __forceinline__ void block_items(const W2BatchDesc *__restrict__ d, int nprob, int nitems,
                                             unsigned long long *ctr, unsigned long long base, int pass,
                                             typename Wave2Tile<float, C, true, false, true, 1, 8, 2>::Lds &lds,
                                             int *sclaim) {
@@ -192,8 +219,8 @@ __device__ __forceinline__ void block_items(const W2BatchDesc<C> *__restrict__ d
     const int it = __builtin_amdgcn_readfirstlane(sclaim[0]);
     const int p = __builtin_amdgcn_readfirstlane(sclaim[1]);
     if (it >= nitems) break;
-    const W2BatchDesc<C> D = uniform_load(d + p);
-    W2Args<float, C> a = D.a;
+    const W2BatchDesc D = uniform_load(d + p);
+    W2Args<float, C> a = w2_args_as<C>(D.a);
     const bool src = a.src && 2 * pass < D.nsrc;  // values past the end are zero padding
     a.src = src ? a.src + 2 * pass : nullptr;
     a.src_k = src ? a.src_k : -1;
@@ -209,7 +236,7 @@ __device__ __forceinline__ void block_items(const W2BatchDesc<C> *__restrict__ d
 // queue values spill); MINW = 1 keeps every value in registers, one workgroup per CU.
 template <typename C, int MINW>
 __global__ __launch_bounds__(kW2Lanes) __attribute__((amdgpu_waves_per_eu(MINW))) void batch_w2_kernel(
-    const W2BatchDesc<C> *__restrict__ d, int nprob, int nitems, unsigned long long *ctr, unsigned long long base,
+    const W2BatchDesc *__restrict__ d, int nprob, int nitems, unsigned long long *ctr, unsigned long long base,
     int pass) {
   __shared__ typename Wave2Tile<float, C, true, false, true, 1, 8, 2>::Lds lds;
   __shared__ int sclaim[2];
@@ -221,7 +248,7 @@ __global__ __launch_bounds__(kW2Lanes) __attribute__((amdgpu_waves_per_eu(MINW))
 // (even p > 0) of d, or table 3 for every block when tuning.
 template <typename C, int MINW>
 __global__ __launch_bounds__(kW2Lanes) __attribute__((amdgpu_waves_per_eu(MINW))) void batch_w2_run_kernel(
-    const W2BatchDesc<C> *__restrict__ d, int nprob, int nitems, unsigned long long *ctr, int npasses, int tuning) {
+    const W2BatchDesc *__restrict__ d, int nprob, int nitems, unsigned long long *ctr, int npasses, int tuning) {
   __shared__ typename Wave2Tile<float, C, true, false, true, 1, 8, 2>::Lds lds;
   __shared__ int sclaim[2];
   cooperative_groups::grid_group grid = cooperative_groups::this_grid();
@@ -235,23 +262,18 @@ __global__ __launch_bounds__(kW2Lanes) __attribute__((amdgpu_waves_per_eu(MINW))
 
 typedef void (*batch_fn)(const BatchDesc *, int, int, unsigned long long *, unsigned long long, int);
 typedef void (*batch_run_fn)(const BatchDesc *, int, int, unsigned long long *, int);
-typedef void (*w2_fn)(const W2BatchDesc<float> *, int, int, unsigned long long *, unsigned long long, int);
-typedef void (*w2_run_fn)(const W2BatchDesc<float> *, int, int, unsigned long long *, int, int);
+typedef void (*w2_fn)(const W2BatchDesc *, int, int, unsigned long long *, unsigned long long, int);
+typedef void (*w2_run_fn)(const W2BatchDesc *, int, int, unsigned long long *, int, int);
 
-// two-step kernels by register cap; the bf16 instantiation reads the same table
-// bytes (W2BatchDesc<bf16> and W2BatchDesc<float> differ in the roc2 pointer's type only)
+// two-step kernels by register cap and roc2 storage type
 constexpr int kW2Minw[] = {1, 4};
 constexpr int kNumW2 = 2;
 w2_fn w2_kernel(bool bf16, int v) {
-  if (bf16)
-    return v ? reinterpret_cast<w2_fn>(&batch_w2_kernel<__hip_bfloat16, 4>)
-             : reinterpret_cast<w2_fn>(&batch_w2_kernel<__hip_bfloat16, 1>);
+  if (bf16) return v ? &batch_w2_kernel<__hip_bfloat16, 4> : &batch_w2_kernel<__hip_bfloat16, 1>;
   return v ? &batch_w2_kernel<float, 4> : &batch_w2_kernel<float, 1>;
 }
 w2_run_fn w2_run_kernel(bool bf16, int v) {
-  if (bf16)
-    return v ? reinterpret_cast<w2_run_fn>(&batch_w2_run_kernel<__hip_bfloat16, 4>)
-             : reinterpret_cast<w2_run_fn>(&batch_w2_run_kernel<__hip_bfloat16, 1>);
+  if (bf16) return v ? &batch_w2_run_kernel<__hip_bfloat16, 4> : &batch_w2_run_kernel<__hip_bfloat16, 1>;
   return v ? &batch_w2_run_kernel<float, 4> : &batch_w2_run_kernel<float, 1>;
 }
 
@@ -292,8 +314,8 @@ struct gr_batch {
   // two-step blocks: tables [0] first block of a call, [1] odd blocks, [2] even blocks
   // after the first, [3] tuning (reads the fields, writes the spares); [4] the single
   // step that ends an odd step count
-  W2BatchDesc<float> *dw2 = nullptr;
-  std::vector<W2BatchDesc<float>> hw2;
+  W2BatchDesc *dw2 = nullptr;
+  std::vector<W2BatchDesc> hw2;
   int w2_zchunk = 0;   // z planes per two-step item (0 = whole grid)
   int w2_variant = 0;  // index into kW2Minw
   int w2_items = 0, w2_grid = 0, w2_occupancy[kNumW2] = {};
@@ -398,14 +420,14 @@ int build_tables(gr_batch *b, bool spares, int npasses = 0) {
 // problems' fields and write their spares (no source).
 int build_w2_tables(gr_batch *b) {
   const int n = (int)b->probs.size();
-  b->hw2.assign(4 * (size_t)n, W2BatchDesc<float>{});
+  b->hw2.assign(4 * (size_t)n, W2BatchDesc{});
   int item = 0;
   for (int oi = 0; oi < n; ++oi) {
     gr_handle *h = b->probs[b->order[oi]];
     const Problem &P = gr_internal_problem(h);
-    W2BatchDesc<float> D{};
+    W2BatchDesc D{};
     W2Args<float, float> &a = D.a;
-    a.roc2 = (const float *)P.roc2;  // a pointer either way: bf16 tables are read as W2BatchDesc<bf16>
+    a.roc2 = (const float *)P.roc2;  // the field's address; the kernel reads it as its storage type
     for (int r = 0; r < 5; ++r) a.c[r] = (float)P.c[r];
     a.csum = fast_csum(P.c);
     a.sz = (long long)P.pitch * P.rows;
@@ -422,7 +444,7 @@ int build_w2_tables(gr_batch *b) {
     const long long done = gr_internal_napplied(h);
     const bool has_src = src && sidx >= 0 && done < nsrc;
     for (int t = 0; t < 4; ++t) {
-      W2BatchDesc<float> E = D;
+      W2BatchDesc E = D;
       void *buf[4];
       gr_internal_fused_buffers(h, t < 3 ? t : 0, buf);
       E.a.P = (const float *)buf[0];  // u^(n-1)
@@ -448,7 +470,7 @@ int build_w2_tables(gr_batch *b) {
   b->w2_items = item;
   b->w2_grid = std::max(1, std::min(item, b->ncu * std::max(1, b->w2_occupancy[b->w2_variant])));
   b->w2_run_grid = std::max(1, std::min(item, b->ncu * std::max(1, b->w2_run_occupancy[b->w2_variant])));
-  hipError_t e = hipMemcpyAsync(b->dw2, b->hw2.data(), sizeof(W2BatchDesc<float>) * b->hw2.size(),
+  hipError_t e = hipMemcpyAsync(b->dw2, b->hw2.data(), sizeof(W2BatchDesc) * b->hw2.size(),
                                 hipMemcpyHostToDevice, b->stream);
   if (e != hipSuccess) return set_errorf(GR_EHIP, "gr_batch: descriptor copy: %s", hipGetErrorString(e));
   return GR_OK;
@@ -469,7 +491,7 @@ hipError_t enqueue_steps(gr_batch *b, int table, int napply, int tblock, bool tu
   if (tblock == 2) {
     for (int pass = 0; 2 * pass + 1 < napply; ++pass, s += 2) {
       const int t = tuning ? 3 : pass == 0 ? 0 : (pass & 1) ? 1 : 2;
-      const W2BatchDesc<float> *tab = b->dw2 + (size_t)t * n;
+      const W2BatchDesc *tab = b->dw2 + (size_t)t * n;
       hipLaunchKernelGGL(w2_kernel(b->bf16, b->w2_variant), dim3(b->w2_grid), dim3(kW2Lanes), 0, st, tab, n,
                          b->w2_items, b->ctr, base, pass);
       base += (unsigned long long)(b->w2_items + b->w2_grid);
@@ -502,7 +524,7 @@ hipError_t enqueue_run(gr_batch *b, int table, int napply, int tblock, bool tuni
     return hipLaunchCooperativeKernel(f, dim3(b->run_grid), dim3(64 * V.bx, V.by), args, 0, st);
   }
   int npasses = napply / 2, tun = tuning ? 1 : 0, items = b->w2_items;
-  const W2BatchDesc<float> *tab = b->dw2;
+  const W2BatchDesc *tab = b->dw2;
   void *args[] = {&tab, &n, &items, &ctr, &npasses, &tun};
   const void *f = reinterpret_cast<const void *>(w2_run_kernel(b->bf16, b->w2_variant));
   e = hipLaunchCooperativeKernel(f, dim3(b->w2_run_grid), dim3(kW2Lanes), args, 0, st);
@@ -646,7 +668,7 @@ extern "C" int gr_batch_create(const gr_desc *descs, int n, gr_batch **out) {
   if (rc == GR_OK) {
     int dev = 0;
     e = hipMalloc(&b->ddesc, sizeof(BatchDesc) * 4 * (size_t)n);
-    if (e == hipSuccess) e = hipMalloc(&b->dw2, sizeof(W2BatchDesc<float>) * 4 * (size_t)n);
+    if (e == hipSuccess) e = hipMalloc(&b->dw2, sizeof(W2BatchDesc) * 4 * (size_t)n);
     if (e == hipSuccess) e = hipMalloc(&b->ctr, sizeof(unsigned long long));
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreate(&b->ev0);
