@@ -81,13 +81,28 @@ __global__ void grad_segment_kernel(T *g, const T *__restrict__ nus, const T *__
   const bool at_src = src && o == src_idx;
   double x2 = (double)xs[(long long)(nsteps + 1) * fstride + o];  // x_{a+2} for a = a1 - 1
   double x1 = (double)xs[(long long)nsteps * fstride + o];
-  for (int t = nsteps - 1; t >= 0; --t) {  // t = a - a0
-    const double x0 = (double)xs[(long long)t * fstride + o];
-    double d = (x2 - x1) - (x1 - x0);
-    if (at_src && a0 + t < nsrc) d = d - (double)src[a0 + t];
-    acc = (T)((double)acc + (double)nus[(long long)t * fstride + o] * d);
-    x2 = x1;
-    x1 = x0;
+  // the loads of QB steps are issued together (each cell's terms are still added one
+  // step at a time, in the walk's order)
+  constexpr int QB = 8;
+  for (int t0 = nsteps - 1; t0 >= 0; t0 -= QB) {  // t = a - a0
+    T xq[QB], nq[QB];
+#pragma unroll
+    for (int q = 0; q < QB; ++q)
+      if (t0 - q >= 0) {
+        xq[q] = __builtin_nontemporal_load(xs + (long long)(t0 - q) * fstride + o);
+        nq[q] = __builtin_nontemporal_load(nus + (long long)(t0 - q) * fstride + o);
+      }
+#pragma unroll
+    for (int q = 0; q < QB; ++q)
+      if (t0 - q >= 0) {
+        const int t = t0 - q;
+        const double x0 = (double)xq[q];
+        double d = (x2 - x1) - (x1 - x0);
+        if (at_src && a0 + t < nsrc) d = d - (double)src[a0 + t];
+        acc = (T)((double)acc + (double)nq[q] * d);
+        x2 = x1;
+        x1 = x0;
+      }
   }
   gc = acc;
 }

@@ -21,6 +21,11 @@ of gr_batch_set_config and gr_batch_set_time_block, or the batch autotuner; one
 persistent launch per step (TBLOCK 1) or per two-step block (2), replayed from a HIP
 graph (--mode 1, the default), launched one by one (--mode 0), or all in one
 cooperative launch (--mode 2).
+
+--shot N,NT,CKPT runs one seismic shot instead (girih_amd.adjoint; 8-layer model,
+15 Hz Ricker at 50 m depth, receivers at z = 8 cells): the forward run and the
+vector-Jacobian product of a seeded cotangent, twice, timing the second
+(--shot-prec f32, f64 or mixed).
 """
 import argparse
 import json
@@ -58,9 +63,56 @@ def batch_main(a):
                           "glups": cells * a.steps / s["last_ms"] / 1e6}))
 
 
+def shot_main(a):
+    """One shot (girih_amd.adjoint): forward traces, then the vector-Jacobian product of
+    a seeded cotangent; FP32 (GR_FAST), FP64 or mixed precision."""
+    import time
+
+    import numpy as np
+    import torch
+
+    from . import adjoint
+    n, nt, ckpt = (int(x) for x in a.shot.split(","))
+    h = 10.0
+    dt = physics.cfl_dt(4500.0 * 1.05, h, 0.4)
+    src = physics.roc2_const(1500.0, dt, h) * physics.ricker(15.0, dt, nt)
+    pt = (4 + n // 2, 4 + n // 2, 4 + 5)
+    with gpu.Stepper(0, n, n, n, dtype=gpu.GR_F64, coef=physics.fd8_coef()) as st:
+        st.init_roc2_model(gpu.vmodel(7, dt / h))
+        m0 = np.ascontiguousarray((dt / h) ** 2 / st.download_roc2()[4:-4, 4:-4, 4:4 + n])
+    if a.shot_prec == "mixed":
+        shot = adjoint.MixedShot(n, n, n, nt, dt / h, pt, src, 4 + 8, ckpt=ckpt)
+        m = torch.tensor(m0, device="cuda")
+    else:
+        dtype = gpu.GR_F32 if a.shot_prec == "f32" else gpu.GR_F64
+        if dtype == gpu.GR_F32:
+            src = src.astype(np.float32).astype(np.float64)
+        shot = adjoint.Shot(n, n, n, nt, dt / h, pt, src, 4 + 8, ckpt=ckpt, dtype=dtype)
+        m = torch.tensor(m0, dtype=shot.torch_dtype, device="cuda")
+    res = None
+    times = []
+    for _ in range(2):  # the first round loads the code objects and allocates the backward buffers
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        tr = shot.forward(m)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        if res is None:
+            g = torch.Generator(device="cuda").manual_seed(11)
+            res = torch.randn(tr.shape, generator=g, device="cuda", dtype=tr.dtype)
+        shot.vjp(m, res)
+        torch.cuda.synchronize()
+        times.append((t1 - t0, time.perf_counter() - t1))
+    shot.close()
+    print(json.dumps({"shot": [n, n, n], "steps": nt, "checkpoint_every": ckpt, "precision": a.shot_prec,
+                      "forward_s": round(times[-1][0], 4), "backward_s": round(times[-1][1], 4)}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--batch", action="store_true", help="the 16-problem bf16 wave batch")
+    ap.add_argument("--shot", default=None, help="N,NT,CKPT: an N^3 shot of NT steps, forward and adjoint")
+    ap.add_argument("--shot-prec", choices=["f32", "f64", "mixed"], default="f32")
     ap.add_argument("--kernel", type=int)
     ap.add_argument("--n", type=int)
     ap.add_argument("--shape", default=None, help="NX,NY,NZ (instead of --n for all three)")
@@ -81,6 +133,8 @@ def main():
     a = ap.parse_args()
     if a.batch:
         return batch_main(a)
+    if a.shot:
+        return shot_main(a)
     if a.kernel is None or (a.n is None and a.shape is None):
         ap.error("--kernel and --n (or --shape) are required (or --batch)")
     nx, ny, nz = (int(x) for x in a.shape.split(",")) if a.shape else (a.n, a.n, a.n)
