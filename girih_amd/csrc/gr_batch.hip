@@ -1,6 +1,6 @@
 // gr_batch.hip -- many independent kernel-0 wave problems of uneven size, one
-// persistent launch per time step, the step loop optionally replayed from a HIP
-// graph.
+// persistent launch per time step (or per two-step block), the step loop optionally
+// replayed from a HIP graph or run as one cooperative launch.
 //
 // girih steps one grid at a time and balances its CPU thread groups with a
 // dynamic ready queue of diamond tiles (src/kernels/diamond_ts.c:663-747,
@@ -14,7 +14,8 @@
 //
 // The claim counter is never reset between the steps of one call: step s claims
 // values in [s*(items+grid), (s+1)*(items+grid)), because every workgroup makes
-// exactly one failing claim per step.  One reset kernel (not a memset, so the
+// exactly one failing claim per step (launches of two-step blocks and single steps
+// each add their own items + grid).  One reset kernel (not a memset, so the
 // sequence can live in a HIP graph) starts each call.
 //
 // Fields are FP32 with GR_FAST arithmetic; roc2 is a per-cell field stored as

@@ -216,8 +216,10 @@ int gr_download_roc2(gr_handle *h, const int ldomain_shape[3], void *U3);
 /* ---- Batched problems (no counterpart in girih, which steps one grid) ----
  * n independent kernel-0 problems (GR_F32 | GR_FAST, roc2 field, optionally
  * GR_COEF_BF16; same flags for all) of any sizes advance together: each time step
- * is ONE persistent launch whose workgroups claim (problem, xy tile, z chunk)
- * items from a device work queue, largest problem first.  Every problem keeps
+ * (or two-step block, gr_batch_set_time_block) is ONE persistent launch whose
+ * workgroups claim (problem, xy tile, z chunk) items from a device work queue,
+ * largest problem first, or a whole call is one cooperative launch with a grid
+ * barrier between steps (gr_batch_step mode 2).  Every problem keeps
  * its own handle (gr_batch_problem) for uploads, models, sources, downloads and
  * comparisons; its results equal gr_step's on the same handle. */
 typedef struct gr_batch gr_batch;
