@@ -25,6 +25,7 @@
 #include <hip/hip_cooperative_groups.h>
 
 #include <algorithm>
+#include <climits>
 #include <numeric>
 #include <vector>
 
@@ -53,6 +54,41 @@ struct BatchDesc {
   int pad;
 };
 static_assert(sizeof(BatchDesc) % 8 == 0, "descriptor words");
+
+// A problem's receiver plane: after each application the interior of plane rec_k of
+// the field the step wrote goes to trace[a][j][i] (a = the problem's application
+// count, gr_shot_forward's layout), for a < max_steps.
+struct RecDesc {
+  float *trace;
+  long long plane;  // rec_k * plane size: offset of the plane in the field
+  long long sy;
+  int prob;         // index of the problem in the descriptor tables
+  int nx, ny;
+  int a0;           // applications before this call
+  int max_steps;
+  int row0;         // first row of this receiver in the rows of all receivers
+};
+static_assert(sizeof(RecDesc) % 8 == 0, "descriptor words");
+
+// Row `row` of all receivers' planes after call step `step`, from the fields the step
+// wrote (table d of that step's parity); lanes stride over x.
+__device__ __forceinline__ void record_row(const BatchDesc *__restrict__ d, const RecDesc *__restrict__ r, int nrec,
+                                           int row, int step, int lane, int nlanes) {
+  int ri = 0;
+  while (ri + 1 < nrec && r[ri + 1].row0 <= row) ++ri;
+  const RecDesc R = r[ri];
+  const int j = row - R.row0, a = R.a0 + step;
+  if (j >= R.ny || a >= R.max_steps) return;
+  const float *u = d[R.prob].u + R.plane + (long long)(j + 4) * R.sy + 4;
+  float *t = R.trace + ((long long)a * R.ny + j) * R.nx;
+  for (int i = lane; i < R.nx; i += nlanes) t[i] = u[i];
+}
+
+// The receivers of one step (launches per step): one workgroup per row.
+__global__ __launch_bounds__(256) void batch_record_kernel(const BatchDesc *__restrict__ d,
+                                                           const RecDesc *__restrict__ r, int nrec, int step) {
+  record_row(d, r, nrec, blockIdx.x, step, threadIdx.x, blockDim.x);
+}
 
 constexpr int kMaxBatch = 256;
 
@@ -142,17 +178,26 @@ __global__ __launch_bounds__(64 * BX * BY) void batch_kernel(const BatchDesc *__
 // acquire write back and invalidate each XCD's L2 once per step, the maintenance
 // a launch boundary does, so the next step reads what every XCD wrote.  Step s
 // reads table d + (s & 1) * nprob; its claims start at s * (nitems + gridDim.x).
+//
+// With receivers, step s's planes are recorded after its barrier, while step s+1
+// runs: step s+1 writes the other field of each pair, and the barrier after it
+// orders the recording before step s+2 overwrites the recorded field.
 template <typename C, int BX, int BY, int RY, bool RING>
 __global__ __launch_bounds__(64 * BX * BY) void batch_run_kernel(const BatchDesc *__restrict__ d, int nprob,
-                                                                 int nitems, unsigned long long *ctr, int napply) {
+                                                                 int nitems, unsigned long long *ctr, int napply,
+                                                                 const RecDesc *__restrict__ rec, int nrec,
+                                                                 int nrows) {
   using ST = SweepTile<float, C, 0, BY, RY, true, false, RING, kFastND, BX, true>;
   __shared__ typename ST::Lds lds;
   __shared__ int sclaim[2];
   cooperative_groups::grid_group grid = cooperative_groups::this_grid();
   const unsigned long long per = (unsigned long long)nitems + gridDim.x;
+  const int lane = threadIdx.y * blockDim.x + threadIdx.x, nlanes = blockDim.x * blockDim.y;
   for (int s = 0; s < napply; ++s) {
     step_items<C, BX, BY, RY, RING>(d + (size_t)(s & 1) * nprob, nprob, nitems, ctr, per * s, s, lds, sclaim);
     grid.sync();
+    for (int row = blockIdx.x; row < nrows; row += gridDim.x)
+      record_row(d + (size_t)(s & 1) * nprob, rec, nrec, row, s, lane, nlanes);
   }
 }
 
@@ -262,7 +307,7 @@ __global__ __launch_bounds__(kW2Lanes) __attribute__((amdgpu_waves_per_eu(MINW))
 }
 
 typedef void (*batch_fn)(const BatchDesc *, int, int, unsigned long long *, unsigned long long, int);
-typedef void (*batch_run_fn)(const BatchDesc *, int, int, unsigned long long *, int);
+typedef void (*batch_run_fn)(const BatchDesc *, int, int, unsigned long long *, int, const RecDesc *, int, int);
 typedef void (*w2_fn)(const W2BatchDesc *, int, int, unsigned long long *, unsigned long long, int);
 typedef void (*w2_run_fn)(const W2BatchDesc *, int, int, unsigned long long *, int, int);
 
@@ -326,6 +371,12 @@ struct gr_batch {
   int run_occupancy[kNumVariants] = {}, w2_run_occupancy[kNumW2] = {};
   int run_grid = 0, w2_run_grid = 0;
   int last_mode = 0, last_grid = 0;
+  // receiver planes (gr_batch_set_receivers), per problem in the caller's order
+  std::vector<int> rec_k, rec_max;
+  std::vector<float *> rec_trace;
+  RecDesc *drec = nullptr;
+  std::vector<RecDesc> hrec;
+  int nrows = 0;
   unsigned long long *ctr = nullptr;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -405,6 +456,30 @@ int build_tables(gr_batch *b, bool spares, int npasses = 0) {
         b->hdesc[(size_t)(2 + q) * n + oi] = S;
       }
     }
+  }
+  b->hrec.clear();
+  b->nrows = 0;
+  for (int oi = 0; oi < n && !spares; ++oi) {  // receivers of the problems (tuning records nothing)
+    const int i = b->order[oi];
+    if (!b->rec_trace[i]) continue;
+    const Problem &P = gr_internal_problem(b->probs[i]);
+    RecDesc R{};
+    R.trace = b->rec_trace[i];
+    R.plane = (long long)b->rec_k[i] * P.pitch * P.rows;
+    R.sy = P.pitch;
+    R.prob = oi;
+    R.nx = P.nx;
+    R.ny = P.ny;
+    R.a0 = (int)std::min<long long>(gr_internal_napplied(b->probs[i]), INT32_MAX);
+    R.max_steps = b->rec_max[i];
+    R.row0 = b->nrows;
+    b->nrows += P.ny;
+    b->hrec.push_back(R);
+  }
+  if (!b->hrec.empty()) {
+    hipError_t e = hipMemcpyAsync(b->drec, b->hrec.data(), sizeof(RecDesc) * b->hrec.size(), hipMemcpyHostToDevice,
+                                  b->stream);
+    if (e != hipSuccess) return set_errorf(GR_EHIP, "gr_batch: receiver copy: %s", hipGetErrorString(e));
   }
   b->nitems = item;
   b->grid = std::max(1, std::min(item, b->ncu * std::max(1, b->occupancy[b->variant])));
@@ -499,10 +574,12 @@ hipError_t enqueue_steps(gr_batch *b, int table, int napply, int tblock, bool tu
     }
     table = 0;
   }
+  const int nrec = table == 0 ? (int)b->hrec.size() : 0;
   for (int q = 0; s < napply; ++s, ++q) {
     const BatchDesc *tab = b->ddesc + (size_t)(table + (q & 1)) * n;
     hipLaunchKernelGGL(f, dim3(b->grid), dim3(64 * V.bx, V.by), 0, st, tab, n, b->nitems, b->ctr, base, s);
     base += (unsigned long long)(b->nitems + b->grid);
+    if (nrec) hipLaunchKernelGGL(batch_record_kernel, dim3(b->nrows), dim3(256), 0, st, tab, b->drec, nrec, s);
   }
   return hipGetLastError();
 }
@@ -520,7 +597,9 @@ hipError_t enqueue_run(gr_batch *b, int table, int napply, int tblock, bool tuni
   if (tblock != 2) {
     const BatchDesc *tab = b->ddesc + (size_t)table * n;
     int items = b->nitems;
-    void *args[] = {&tab, &n, &items, &ctr, &napply};
+    const RecDesc *rec = b->drec;
+    int nrec = table == 0 ? (int)b->hrec.size() : 0, nrows = nrec ? b->nrows : 0;
+    void *args[] = {&tab, &n, &items, &ctr, &napply, &rec, &nrec, &nrows};
     const void *f = reinterpret_cast<const void *>(b->bf16 ? V.runbf16 : V.run32);
     return hipLaunchCooperativeKernel(f, dim3(b->run_grid), dim3(64 * V.bx, V.by), args, 0, st);
   }
@@ -611,6 +690,7 @@ int run_steps(gr_batch *b, int table, int napply, int mode, int tblock = 1, bool
 // spares allocated and holding that halo).
 int fused_ready(gr_batch *b, bool *ok) {
   *ok = b->w2_occupancy[b->w2_variant] >= 1;
+  for (float *t : b->rec_trace) *ok = *ok && !t;  // receivers are recorded after single steps only
   for (gr_handle *h : b->probs) {
     if (!*ok) break;
     const int rc = gr_internal_fused_prepare(h, ok);
@@ -636,6 +716,7 @@ extern "C" int gr_batch_destroy(gr_batch *b) {
   for (gr_handle *h : b->probs) gr_destroy(h);
   if (b->ddesc) (void)hipFree(b->ddesc);
   if (b->dw2) (void)hipFree(b->dw2);
+  if (b->drec) (void)hipFree(b->drec);
   if (b->ctr) (void)hipFree(b->ctr);
   if (b->ev0) (void)hipEventDestroy(b->ev0);
   if (b->ev1) (void)hipEventDestroy(b->ev1);
@@ -659,6 +740,9 @@ extern "C" int gr_batch_create(const gr_desc *descs, int n, gr_batch **out) {
   }
   gr_batch *b = new gr_batch();
   b->bf16 = (flags0 & GR_COEF_BF16) != 0;
+  b->rec_k.assign(n, 0);
+  b->rec_max.assign(n, 0);
+  b->rec_trace.assign(n, nullptr);
   int rc = GR_OK;
   for (int i = 0; i < n && rc == GR_OK; ++i) {
     gr_handle *h = nullptr;
@@ -670,6 +754,7 @@ extern "C" int gr_batch_create(const gr_desc *descs, int n, gr_batch **out) {
     int dev = 0;
     e = hipMalloc(&b->ddesc, sizeof(BatchDesc) * 4 * (size_t)n);
     if (e == hipSuccess) e = hipMalloc(&b->dw2, sizeof(W2BatchDesc) * 4 * (size_t)n);
+    if (e == hipSuccess) e = hipMalloc(&b->drec, sizeof(RecDesc) * (size_t)n);
     if (e == hipSuccess) e = hipMalloc(&b->ctr, sizeof(unsigned long long));
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreate(&b->ev0);
@@ -780,6 +865,20 @@ extern "C" int gr_batch_set_time_block(gr_batch *b, int tblock, int block_varian
     return set_error(GR_EUNSUPPORTED, "gr_batch_set_time_block: the two-step kernel does not fit a CU");
   b->tblock = tblock;
   b->w2_variant = wv;
+  return GR_OK;
+}
+
+extern "C" int gr_batch_set_receivers(gr_batch *b, int i, int rec_k, void *trace_dev, int max_steps) {
+  if (!b) return set_error(GR_EINVAL, "gr_batch_set_receivers: null batch");
+  if (i < 0 || i >= (int)b->probs.size()) return set_errorf(GR_EINVAL, "gr_batch_set_receivers: no problem %d", i);
+  const Problem &P = gr_internal_problem(b->probs[i]);
+  if (trace_dev && (rec_k < P.R || rec_k >= P.nz + P.R))
+    return set_errorf(GR_EINVAL, "gr_batch_set_receivers: plane %d is not an interior plane", rec_k);
+  if (trace_dev && max_steps < 0) return set_error(GR_EINVAL, "gr_batch_set_receivers: negative max_steps");
+  drop_graph(b);  // the launch sequence records another set of planes
+  b->rec_trace[i] = (float *)trace_dev;
+  b->rec_k[i] = trace_dev ? rec_k : 0;
+  b->rec_max[i] = trace_dev ? max_steps : 0;
   return GR_OK;
 }
 

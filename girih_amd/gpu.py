@@ -72,7 +72,7 @@ EXPORTS = ["gr_create", "gr_destroy", "gr_last_error", "gr_upload", "gr_download
            "gr_set_graph", "gr_autotune",
            "gr_init_roc2_model",
            "gr_download_roc2", "gr_batch_create", "gr_batch_destroy", "gr_batch_problem", "gr_batch_set_config",
-           "gr_batch_step", "gr_batch_set_time_block", "gr_batch_autotune", "gr_batch_get_stats", "gr_shot_create", "gr_shot_destroy",
+           "gr_batch_step", "gr_batch_set_time_block", "gr_batch_set_receivers", "gr_batch_autotune", "gr_batch_get_stats", "gr_shot_create", "gr_shot_destroy",
            "gr_set_roc2_from_slowness2", "gr_shot_forward", "gr_shot_adjoint", "gr_shot_take_checkpoints"]
 
 _lib = None
@@ -124,6 +124,7 @@ def lib():
             "gr_batch_set_config": ([P, I, I], I),
             "gr_batch_step": ([P, I, I], I),
             "gr_batch_set_time_block": ([P, I, I], I),
+            "gr_batch_set_receivers": ([P, I, I, P, I], I),
             "gr_batch_autotune": ([P, I, I, IP, IP], I),
             "gr_batch_get_stats": ([P, ctypes.POINTER(BatchStats)], I),
             "gr_shot_create": ([P, I, I, I, ctypes.c_double, ctypes.POINTER(P)], I),
@@ -382,6 +383,14 @@ class Batch:
         """1: one launch per step; 2: two-step blocks, block_variant 0 (registers
         uncapped) or 1 (capped at 128) (gr_batch_set_time_block)."""
         check(lib().gr_batch_set_time_block(self._b, int(tblock), int(block_variant)))
+
+    def set_receivers(self, i, rec_k, trace_ptr, max_steps):
+        """Record plane rec_k of problem i after every application into the device
+        array at trace_ptr, FP32 [max_steps][ny][nx] (gr_batch_set_receivers); a torch
+        tensor is taken by its data pointer, None removes the receiver."""
+        ptr = trace_ptr.data_ptr() if hasattr(trace_ptr, "data_ptr") else trace_ptr
+        check(lib().gr_batch_set_receivers(self._b, int(i), int(rec_k), ctypes.c_void_p(ptr) if ptr else None,
+                                           int(max_steps)))
 
     def autotune(self, steps=4, use_graph=False, mode=None):
         v, z = ctypes.c_int(0), ctypes.c_int(0)

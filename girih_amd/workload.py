@@ -42,11 +42,16 @@ def batch_main(a):
     src = physics.roc2_const(3000.0, dt, h) * physics.ricker(15.0, dt, a.steps + 16)
     cells = sum(x * y * z for x, y, z in sizes)
     with gpu.Batch(sizes, flags=gpu.GR_FAST | gpu.GR_COEF_BF16, coef=physics.fd8_coef()) as b:
+        traces = []
         for i, p in enumerate(b.problems):
             p.init_fields(2)
             p.init_roc2_model(gpu.vmodel(2024 + i, dt / h))
             nx, ny, nz = sizes[i]
             p.set_source(4 + nx // 2, 4 + ny // 2, 4 + nz // 2, src)
+            if a.receivers:  # a receiver plane 8 cells below the top, every application
+                import torch
+                traces.append(torch.empty((a.steps + 2, ny, nx), dtype=torch.float32, device="cuda"))
+                b.set_receivers(i, 4 + 8, traces[-1], a.steps + 2)
         if a.config:
             cfg = [int(x) for x in a.config.split(",")] + [1, -1]
             b.set_config(cfg[0], cfg[1])
@@ -59,7 +64,7 @@ def batch_main(a):
         print(json.dumps({"batch": len(sizes), "cells": cells,
                           "config": "%d,%d,%d,%d" % (s["variant"], s["zchunk"], s["time_block"], s["block_variant"]),
                           "tile": [s["tile_x"], s["tile_y"]], "steps": a.steps, "launches": s["last_launches"],
-                          "mode": s["last_mode"], "grid": s["grid"],
+                          "mode": s["last_mode"], "grid": s["grid"], "receivers": a.receivers,
                           "glups": cells * a.steps / s["last_ms"] / 1e6}))
 
 
@@ -112,6 +117,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--batch", action="store_true", help="the 16-problem bf16 wave batch")
     ap.add_argument("--shot", default=None, help="N,NT,CKPT: an N^3 shot of NT steps, forward and adjoint")
+    ap.add_argument("--receivers", action="store_true", help="--batch: record a receiver plane of every problem")
     ap.add_argument("--shot-prec", choices=["f32", "f64", "mixed"], default="f32")
     ap.add_argument("--kernel", type=int)
     ap.add_argument("--n", type=int)

@@ -258,6 +258,14 @@ int gr_batch_set_config(gr_batch *b, int variant, int zchunk);
  *      the device has no cooperative launch.
  * Results do not depend on the mode. */
 int gr_batch_step(gr_batch *b, int napply, int use_graph);
+/* Receiver plane of problem i (shots, after nb_naive_ts.c:196's receiver, which girih
+ * leaves disabled): after every application a the batch runs, the interior of plane
+ * rec_k (halo-inclusive z index) of the field the step wrote goes to the device array
+ * trace_dev, FP32 [max_steps][ny][nx], at trace[a] (a counts the problem's applications
+ * since its fields were set, gr_shot_forward's layout); applications a >= max_steps
+ * are not recorded.  trace_dev = NULL removes the receiver.  A batch with receivers
+ * runs single steps (no two-step blocks). */
+int gr_batch_set_receivers(gr_batch *b, int i, int rec_k, void *trace_dev, int max_steps);
 /* Time steps per launch (no counterpart in girih, whose diamonds block time per
  * grid, src/kernels/diamond_ts.c): 1 = single steps; 2 = two-step blocks, where a
  * work item is (problem, 64 x 16 tile, z chunk) advanced two steps on chip, reading

@@ -11,10 +11,14 @@
 * graph replays of the step loop equal eager launches on every replay;
 * one cooperative launch per call (mode 2, grid barrier between steps) equals a
   launch per step, for single steps and two-step blocks;
+* receiver planes recorded by the batch equal a single shot's traces
+  (gr_shot_forward on the same model and source) bit for bit, in every launch mode;
 * against the FP64 CPU oracle, with the roc2 the batch actually holds, the FP32
   batch stays within a relative L2 error of 1e-5;
 * the layered velocity model has the stated layer speeds and bf16 rounding.
 """
+import ctypes
+
 import numpy as np
 import pytest
 
@@ -227,6 +231,53 @@ def test_batch_one_launch_autotune_keeps_fields():
         U1, U2 = _alone(size, flags, 29, i, 3)
         np.testing.assert_array_equal(got[i][0], U1)
         np.testing.assert_array_equal(got[i][1], U2)
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2])
+def test_batch_receivers_equal_shot_traces(mode):
+    """Shots in a batch: each problem's receiver plane, recorded after every
+    application over two calls, equals the traces of gr_shot_forward for the same
+    squared slowness and source; a problem without receivers and a receiver that
+    stops at max_steps leave the rest untouched."""
+    import torch
+    from girih_amd import adjoint
+    sizes = [(40, 24, 20), (32, 40, 28), (56, 16, 24)]
+    nt, rec_k = 12, 4 + 6
+    src = np.linspace(0.8, -0.6, nt)
+    maxs = [nt, 0, 8]
+    print("batch receivers, mode", mode, flush=True)
+    ms, traces = [], []
+    with gpu.Batch(sizes, flags=gpu.GR_FAST, coef=physics.fd8_coef()) as b:
+        for i, (nx, ny, nz) in enumerate(sizes):
+            p = b.problems[i]
+            p.init_fields(2)
+            rng = np.random.default_rng(40 + i)
+            m = torch.tensor((1.0 / rng.uniform(1500.0, 4500.0, (nz, ny, nx)) ** 2).astype(np.float32), device="cuda")
+            gpu.check(gpu.lib().gr_set_roc2_from_slowness2(p._h, ctypes.c_void_p(m.data_ptr()), DT_OVER_H))
+            p.set_source(4 + nx // 2, 4 + ny // 2, 4 + nz // 2, src)
+            t = torch.full((nt, ny, nx), -7.0, dtype=torch.float32, device="cuda")
+            if maxs[i]:
+                b.set_receivers(i, rec_k, t, maxs[i])
+            ms.append(m)
+            traces.append(t)
+        b.set_time_block(2)  # receivers keep the batch on single steps
+        b.step(5, mode=mode)
+        b.step(7, mode=mode)
+        assert b.stats()["last_launches"] == (1 if mode == 2 else 7)
+        got = [t.cpu().numpy() for t in traces]
+    for i, (nx, ny, nz) in enumerate(sizes):
+        if not maxs[i]:
+            assert np.all(got[i] == -7.0)
+            continue
+        shot = adjoint.Shot(nx, ny, nz, nt, DT_OVER_H, (4 + nx // 2, 4 + ny // 2, 4 + nz // 2), src, rec_k, ckpt=nt,
+                            tune=False)
+        try:
+            ref = shot.forward(ms[i]).cpu().numpy()
+        finally:
+            shot.close()
+        k = maxs[i]
+        np.testing.assert_array_equal(got[i][:k], ref[:k], err_msg="problem %d" % i)
+        assert np.all(got[i][k:] == -7.0)
 
 
 def test_batch_time_block_errors():
