@@ -165,3 +165,34 @@ def test_wave_1024_fp32_meets_error_and_speed_targets_after_1000_steps():
     assert c["max_ref"] > 1e-5  # the wave is alive (its peak is about 2e-4 after 1000 steps)
     assert c["rel_l2"] <= 1e-5
     assert ratio >= 1.8
+
+
+def test_batch16_one_launch_equals_graph_replay():
+    """The 16-problem bf16 batch (dims 192..512 from seed 2024, layered models,
+    Ricker sources): 30 steps as one cooperative launch (a grid barrier per step
+    across all resident workgroups) equal 30 graph-replayed launches, every problem
+    and both fields compared on the device (largest |difference| 0)."""
+    rng = np.random.default_rng(2024)
+    sizes = [tuple(int(x) for x in rng.choice(np.arange(192, 513, 16), 3)) for _ in range(16)]
+    h = 10.0
+    dt = physics.cfl_dt(4500.0 * 1.05, h, 0.4)
+    src = physics.roc2_const(3000.0, dt, h) * physics.ricker(15.0, dt, 40)
+    flags = gpu.GR_FAST | gpu.GR_COEF_BF16
+    with gpu.Batch(sizes, flags=flags, coef=physics.fd8_coef()) as g, \
+            gpu.Batch(sizes, flags=flags, coef=physics.fd8_coef()) as one:
+        for b in (g, one):
+            for i, p in enumerate(b.problems):
+                p.init_fields(2)
+                p.init_roc2_model(gpu.vmodel(2024 + i, dt / h))
+                nx, ny, nz = sizes[i]
+                p.set_source(4 + nx // 2, 4 + ny // 2, 4 + nz // 2, src)
+            b.set_config(3, 64)
+        print("batch16: graph replay, then one launch", flush=True)
+        g.step(30, use_graph=True)
+        one.step(30, mode=2)
+        assert one.stats()["last_launches"] == 1 and g.stats()["last_launches"] == 30
+        for i in range(len(sizes)):
+            for f in (0, 1):
+                c = one.problems[i].compare(f, g.problems[i], f)
+                assert c["max_diff"] == 0.0, (i, f, c)
+                assert c["max_ref"] > 0
