@@ -24,7 +24,7 @@ hipError_t launch_sweep_bf16(int kid, bool fast, bool rocs, const SweepArgs<floa
 
 #ifdef GR_LAUNCH_IMPL
 template <typename T, typename C, int KID, bool FAST, bool ROCS, bool NT, int BY, int RY, int MINW, bool RING = false,
-          int BX = 1>
+          int BX = 1, bool PF = false>
 inline hipError_t launch_cfg(const SweepArgs<T, C> &a0, const LaunchCfg &lc, hipStream_t st) {
   SweepArgs<T, C> a = a0;
   const int TX = 64 * BX, TY = BY * RY;
@@ -39,7 +39,7 @@ inline hipError_t launch_cfg(const SweepArgs<T, C> &a0, const LaunchCfg &lc, hip
   // roc2 field (bf16 roc2 at 1024^3: 307 -> 353 GLUP/s).  Scalar-roc2 FP32 (-2%) and
   // FP64 (-11%) sweeps keep SGPR-base global loads.
   constexpr bool BUF = KID == 0 && !ROCS && std::is_same<T, float>::value;
-  hipLaunchKernelGGL((sweep_kernel<T, C, KID, BY, RY, FAST, ROCS, MINW, RING, kFastND, BX, NT, BUF>), dim3(nb),
+  hipLaunchKernelGGL((sweep_kernel<T, C, KID, BY, RY, FAST, ROCS, MINW, RING, kFastND, BX, NT, BUF, PF && KID == 0>), dim3(nb),
                      dim3(TX, BY), 0, st, a);
   return hipGetLastError();
 }
@@ -73,6 +73,7 @@ inline hipError_t launch_variant(const SweepArgs<T, C> &a, const LaunchCfg &lc, 
       case 13: return launch_cfg<T, C, KID, FAST, ROCS, NT, 8, 2, 0, true, 2>(a, lc, st);
       case 14: return launch_cfg<T, C, KID, FAST, ROCS, NT, 4, 4, W, true, 2>(a, lc, st);
       case 15: return launch_cfg<T, C, KID, FAST, ROCS, NT, 2, 4, 0, ring_fits<T, KID, 4, 2, 4>(), 4>(a, lc, st);
+      case 16: return launch_cfg<T, C, KID, FAST, ROCS, NT, 8, 2, 0, true, 2, true>(a, lc, st);
       default: break;
     }
   }

@@ -82,13 +82,14 @@ int streaming_sweep(const Problem &P) {
 //  12: 128x8 (2 x 4) ring, 256 lanes    13: 128x16 (8 x 2) ring, 1024 lanes
 //  14: as 10 with >= 4 / 3 waves per SIMD
 //  15: 256x8 (4 waves wide, 2 x 4) ring (double-buffered tile where the ring does not fit in LDS)
+//  16: kernel 0 only: as 13, with u^(n-1) (and roc2) of plane k+1 loaded while plane k is computed
 int num_tile_variants(int kid, int dtype) {
   (void)dtype;
-  return (kid == 0 || kid == 1 || kid == 5) ? 16 : 1;
+  return kid == 0 ? 17 : (kid == 1 || kid == 5) ? 16 : 1;
 }
 void tile_of_variant(int kid, int dtype, int variant, int *tx, int *ty) {
-  static const int rows[16] = {16, 32, 16, 32, 16, 32, 16, 16, 16, 32, 16, 16, 8, 16, 16, 8};
-  static const int cols[16] = {64, 64, 64, 64, 64, 64, 64, 64, 128, 128, 128, 128, 128, 128, 128, 256};
+  static const int rows[17] = {16, 32, 16, 32, 16, 32, 16, 16, 16, 32, 16, 16, 8, 16, 16, 8, 16};
+  static const int cols[17] = {64, 64, 64, 64, 64, 64, 64, 64, 128, 128, 128, 128, 128, 128, 128, 256, 128};
   const bool ok = variant > 0 && variant < num_tile_variants(kid, dtype);
   *tx = ok ? cols[variant] : 64;
   *ty = ok ? rows[variant] : 16;

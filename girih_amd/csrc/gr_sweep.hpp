@@ -235,8 +235,10 @@ struct Tile {
 
 // BUF: every plane access through a buffer resource (scalar plane base, 32-bit lane
 // offset) instead of global loads; measured per case, see the launch code.
+// PF (kernel 0): the u^(n-1) and roc2 operands of plane k+1 are loaded while plane k
+// is computed, so their latency overlaps a whole plane instead of the LDS exchange.
 template <typename T, typename C, int KID, int BY, int RY, bool FAST, bool ROCS, bool RING, int ND, int BX = 1,
-          bool STRM = false, bool BUF = false>
+          bool STRM = false, bool BUF = false, bool PF = false>
 struct SweepTile;
 
 // MINW > 0 asks the compiler for at least MINW waves per SIMD (caps VGPRs at 512/MINW).
@@ -245,10 +247,10 @@ struct SweepTile;
 // the neighbouring tiles fetch it too and it is still in L2; otherwise the LDS is
 // double buffered and the halo is fetched one plane ahead.
 template <typename T, typename C, int KID, int BY, int RY, bool FAST, bool ROCS, int MINW, bool RING, int ND = kFastND,
-          int BX = 1, bool STRM = false, bool BUF = false>
+          int BX = 1, bool STRM = false, bool BUF = false, bool PF = false>
 __global__ __launch_bounds__(64 * BX * BY) __attribute__((amdgpu_waves_per_eu(MINW > 0 ? MINW : 1)))
 void sweep_kernel(const SweepArgs<T, C> a) {
-  using ST = SweepTile<T, C, KID, BY, RY, FAST, ROCS, RING, ND, BX, STRM, BUF>;
+  using ST = SweepTile<T, C, KID, BY, RY, FAST, ROCS, RING, ND, BX, STRM, BUF, PF>;
   __shared__ typename ST::Lds lds;
   const int nb = a.tiles_x * a.tiles_y * a.nzc;
   const int lb = a.xcd_order ? xcd_remap(blockIdx.x, nb) : (int)blockIdx.x;
@@ -261,7 +263,7 @@ void sweep_kernel(const SweepArgs<T, C> a) {
 // One tile of one sweep: the z-streaming body shared by the single-grid kernel
 // and the batched persistent kernel.
 template <typename T, typename C, int KID, int BY, int RY, bool FAST, bool ROCS, bool RING, int ND, int BX, bool STRM,
-          bool BUF>
+          bool BUF, bool PF>
 struct SweepTile {
   static constexpr int R = kradius(KID);
   static constexpr int TX = 64 * BX, TY = BY * RY;
@@ -272,8 +274,8 @@ struct SweepTile {
 };
 
 template <typename T, typename C, int KID, int BY, int RY, bool FAST, bool ROCS, bool RING, int ND, int BX, bool STRM,
-          bool BUF>
-__device__ __forceinline__ void SweepTile<T, C, KID, BY, RY, FAST, ROCS, RING, ND, BX, STRM, BUF>::run(const SweepArgs<T, C> &a,
+          bool BUF, bool PF>
+__device__ __forceinline__ void SweepTile<T, C, KID, BY, RY, FAST, ROCS, RING, ND, BX, STRM, BUF, PF>::run(const SweepArgs<T, C> &a,
                                                                                    int tile_x, int tile_y, int zc,
                                                                                    Lds &lds) {
   constexpr int Q = 2 * R + 1;
@@ -369,6 +371,26 @@ __device__ __forceinline__ void SweepTile<T, C, KID, BY, RY, FAST, ROCS, RING, N
 #pragma unroll
   for (int r = 0; r < 5; ++r) c[r] = a.c[r];
 
+  // kernel 0's own-cell operands of plane k: u^(n-1) and, with a roc2 field, roc2.  The
+  // roc2 plane is read through a buffer resource (scalar plane base, 32-bit lane
+  // offset): 11% faster for kernel 0 with a bf16 roc2 field at 1024^3.
+  auto load_u0 = [&](int k, T (&u_)[RY], T (&r_)[RY]) {
+    if constexpr (KID == 0) {
+      const auto up = pl(a.uold ? a.uold : a.u, k);
+      const unsigned cbytes = (unsigned)g.sz * (unsigned)sizeof(C);
+#pragma unroll
+      for (int m = 0; m < RY; ++m) {
+        u_[m] = ld_strm(up, off[m]);
+        if constexpr (!ROCS)
+          r_[m] = bload_coef<T, C, STRM>(plane_rsrc(a.roc2 + (long long)k * g.sz, cbytes), off[m] / (unsigned)sizeof(T));
+        else
+          r_[m] = T(0);
+      }
+    }
+  };
+  T uon[RY], r2n[RY];  // PF: operands of the next plane
+  if constexpr (PF) load_u0(z0, uon, r2n);
+
   for (int kb = z0; kb < z1; kb += Q) {
 #pragma unroll
     for (int p = 0; p < Q; ++p) {
@@ -385,28 +407,27 @@ __device__ __forceinline__ void SweepTile<T, C, KID, BY, RY, FAST, ROCS, RING, N
           if (hw[s]) Lh[hl[s]] = hv[s];
       }
 
-      // operands of plane k (arrive while the LDS tile is written and synchronized)
+      // operands of plane k (arrive while the LDS tile is written and synchronized,
+      // or, with PF, while plane k-1 was computed)
       T uo[RY], r2[RY];
       T cf[RY][NC > 0 ? NC : 1];
-      {
-        const auto up = pl(a.uold ? a.uold : a.u, k);
-        const long long pk = (long long)k * g.sz;
-        // The roc2 plane is read through a buffer resource (scalar plane base, 32-bit
-        // lane offset): 11% faster for kernel 0 with a bf16 roc2 field at 1024^3.  The
-        // coefficient arrays of kernels 2-5 keep pointer loads, which measured 8%
-        // faster than seven buffer resources per plane for kernel 5.
-        const unsigned cbytes = (unsigned)g.sz * (unsigned)sizeof(C);
+      if constexpr (PF) {
 #pragma unroll
         for (int m = 0; m < RY; ++m) {
-          if constexpr (KID == 0) {
-            uo[m] = ld_strm(up, off[m]);
-            if constexpr (!ROCS)
-              r2[m] = bload_coef<T, C, STRM>(plane_rsrc(a.roc2 + pk, cbytes), off[m] / (unsigned)sizeof(T));
-          }
+          uo[m] = uon[m];
+          r2[m] = r2n[m];
+        }
+        load_u0(min(k + 1, zlast), uon, r2n);
+      } else {
+        load_u0(k, uo, r2);
+      }
+      if constexpr (NC > 0) {  // pointer loads: 8% faster than seven buffer resources per plane (kernel 5)
+        const long long pk = (long long)k * g.sz;
+#pragma unroll
+        for (int m = 0; m < RY; ++m)
 #pragma unroll
           for (int n = 0; n < NC; ++n)
             cf[m][n] = ld_coef<T, C, STRM>(a.coef + n * a.cstride + pk + off[m] / (unsigned)sizeof(T));
-        }
       }
       // plane k+R+1 into the slot freed by plane k-R, halo ring of plane k+1
       T nq[RY];

@@ -143,7 +143,7 @@ def test_every_tile_variant_bitexact(kernel, prec, strm):
     orc.sweep(kernel, coef, uo, vo, roc2)
     with gpu.Stepper(kernel, nx, ny, nz, dtype=dtype, coef=_consts(coef, kernel)) as st:
         nvar = _num_variants(st)
-        assert nvar == (16 if kernel in (0, 1, 5) else 1)
+        assert nvar == (17 if kernel == 0 else 16 if kernel in (1, 5) else 1)
         st.set_streaming(strm)
         for var in range(nvar):
             for order, zc in ((0, 0), (1, 5)):
