@@ -58,7 +58,7 @@ __global__ __launch_bounds__(NT) void jacobi_tb_kernel(const J1Args<T> a) {
   __shared__ T lds[TL][2][H0 * W0];  // level t's latest planes (t = 0: the input)
 
   const int ntile = a.tiles_x * a.tiles_y;
-  const int lb = a.xcd_order ? xcd_remap(blockIdx.x, ntile * a.nzc) : (int)blockIdx.x;
+  const int lb = tile_order(blockIdx.x, ntile * a.nzc, ntile, a.xcd_order);
   const int tile = lb % ntile;
   const int tx0 = 1 + (tile % a.tiles_x) * TX;  // first output cell (girih indices)
   const int ty0 = 1 + (tile / a.tiles_x) * TY;

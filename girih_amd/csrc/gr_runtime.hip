@@ -19,6 +19,7 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/girih_gpu.h"
@@ -271,9 +272,9 @@ extern "C" int gr_create(const gr_desc *d, gr_handle **out) {
   P.planes = P.nz + 2 * P.R;
   P.field_elems = (long long)P.pitch * P.rows * P.planes;
   h->lc.zchunk = 0;
-  {  // GIRIH_XCD_ORDER=1 starts from XCD-contiguous tile order (the autotuner tries both)
+  {  // GIRIH_XCD_ORDER=1 or 2 starts from that workgroup order (the autotuner tries all three)
     const char *e = getenv("GIRIH_XCD_ORDER");
-    h->lc.xcd_order = (e && e[0] == '1') ? 1 : 0;
+    h->lc.xcd_order = (e && (e[0] == '1' || e[0] == '2')) ? e[0] - '0' : 0;
   }
   h->lc.t_block = d->t_block <= 0 ? 1 : d->t_block;  // 0: single sweeps until gr_autotune picks a depth
   if (h->lc.t_block > 1) {
@@ -713,7 +714,7 @@ extern "C" int gr_set_streaming(gr_handle *h, int mode) {
 
 extern "C" int gr_set_xcd_order(gr_handle *h, int xcd_order) {
   if (!h) return fail(GR_EINVAL, "gr_set_xcd_order: null handle");
-  if (xcd_order != 0 && xcd_order != 1) return fail(GR_EINVAL, "gr_set_xcd_order: 0 or 1");
+  if (xcd_order < 0 || xcd_order > 2) return fail(GR_EINVAL, "gr_set_xcd_order: 0, 1 or 2");
   h->lc.xcd_order = xcd_order;
   return GR_OK;
 }
@@ -753,29 +754,35 @@ extern "C" int gr_autotune(gr_handle *h, int steps, gr_stats *best) {
       if (zc < nz) zchunks.push_back(zc);
   LaunchCfg best_lc = h->lc;
   float best_ms = 1e30f;
-  auto trial = [&](const LaunchCfg &lc) -> int {
-    // one untimed sweep, then `steps` timed sweeps
+  std::vector<std::pair<float, LaunchCfg>> tried;  // sweep trials: time of `steps` sweeps
+  // one untimed sweep, then n timed sweeps
+  auto time_sweeps = [&](const LaunchCfg &lc, int n, float *ms) -> int {
     hipError_t e = sweep_fields(P, a, b, nullptr, -1, lc, h->stream);
     if (e != hipSuccess) return fail(GR_EHIP, "autotune launch: %s", hipGetErrorString(e));
     HIP_TRY(hipEventRecord(h->ev0, h->stream));
-    for (int s = 0; s < steps; ++s) {
+    for (int s = 0; s < n; ++s) {
       e = (s & 1) ? sweep_fields(P, a, b, nullptr, -1, lc, h->stream) : sweep_fields(P, b, a, nullptr, -1, lc, h->stream);
       if (e != hipSuccess) return fail(GR_EHIP, "autotune launch: %s", hipGetErrorString(e));
     }
     HIP_TRY(hipEventRecord(h->ev1, h->stream));
     HIP_TRY(hipEventSynchronize(h->ev1));
+    HIP_TRY(hipEventElapsedTime(ms, h->ev0, h->ev1));
+    return GR_OK;
+  };
+  auto trial = [&](const LaunchCfg &lc) -> int {
     float ms = 0.f;
-    HIP_TRY(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+    if (int r = time_sweeps(lc, steps, &ms)) return r;
+    tried.emplace_back(ms, lc);
     if (ms < best_ms) {
       best_ms = ms;
       best_lc = lc;
     }
     return GR_OK;
   };
-  // stage 1: tile variant x tile order at each z-chunk size of a coarse set;
-  // stage 2: all z-chunk sizes for the best variant and order
+  // stage 1: tile variant x workgroup order at each z-chunk size of a coarse set;
+  // stage 2: all z-chunk sizes and orders for the best variant
   for (int var = 0; var < num_tile_variants(P.kernel, P.dtype); ++var)
-    for (int order = 0; order < 2; ++order)
+    for (int order = 0; order < 3; ++order)
       for (int zc : {0, std::min(128, nz)}) {
         LaunchCfg lc = h->lc;
         lc.variant = var;
@@ -786,11 +793,43 @@ extern "C" int gr_autotune(gr_handle *h, int steps, gr_stats *best) {
       }
   {
     const LaunchCfg base = best_lc;
-    for (int zc : zchunks) {
-      LaunchCfg lc = base;
-      lc.zchunk = zc;
-      if ((rc = trial(lc))) return rc;
+    for (int order = 0; order < 3; ++order)
+      for (int zc : zchunks) {
+        LaunchCfg lc = base;
+        lc.xcd_order = order;
+        lc.zchunk = zc;
+        if ((rc = trial(lc))) return rc;
+      }
+  }
+  // stage 2b: the four fastest distinct sweep configurations again, 4x as many steps,
+  // twice each in alternation; short trials alone do not separate candidates a few
+  // percent apart
+  {
+    std::stable_sort(tried.begin(), tried.end(),
+                     [](const std::pair<float, LaunchCfg> &x, const std::pair<float, LaunchCfg> &y) {
+                       return x.first < y.first;
+                     });
+    std::vector<LaunchCfg> top;
+    for (const auto &t : tried) {
+      bool seen = false;
+      for (const LaunchCfg &c : top)
+        seen = seen || (c.variant == t.second.variant && c.zchunk == t.second.zchunk &&
+                        c.xcd_order == t.second.xcd_order);
+      if (!seen) top.push_back(t.second);
+      if (top.size() == 4) break;
     }
+    std::vector<float> tmin(top.size(), 1e30f);
+    for (int round = 0; round < 2; ++round)
+      for (size_t i = 0; i < top.size(); ++i) {
+        float ms = 0.f;
+        if ((rc = time_sweeps(top[i], 4 * steps, &ms))) return rc;
+        tmin[i] = std::min(tmin[i], ms / 4);
+      }
+    for (size_t i = 0; i < top.size(); ++i)
+      if (i == 0 || tmin[i] < best_ms) {
+        best_ms = tmin[i];
+        best_lc = top[i];
+      }
   }
   // stage 3: fused passes (time per application) on the scratch copies; kernel 0
   // reads the fields themselves (never written here) and writes the scratch pair
@@ -804,7 +843,7 @@ extern "C" int gr_autotune(gr_handle *h, int steps, gr_stats *best) {
       const int tl = depths[i];
       const int npass = (steps + tl - 1) / tl;
       for (int zc : zchunks)
-      for (int order = 0; order < 2; ++order) {  // XCD-contiguous order keeps the per-level re-reads in one L2
+      for (int order = 0; order < 3; ++order) {  // XCD-contiguous orders keep the per-level re-reads in one L2
         LaunchCfg lc = best_lc;
         lc.t_block = tl;
         lc.zchunk = zc;

@@ -80,7 +80,7 @@ struct SweepArgs {
   T roc2s;             // scalar roc2
   Geom g;
   int tiles_x, tiles_y, nzc, zchunk;
-  int xcd_order;       // 1: renumber workgroups so each XCD owns a contiguous tile range
+  int xcd_order;       // workgroup -> tile order (tile_order): 0 plain, 1 XCD runs, 2 XCD bands per z layer
   const T *src;        // point source value, added at plane src_k, in-plane offset src_off
   int src_k;           // -1 = no source
   int src_off;
@@ -142,6 +142,21 @@ __device__ __forceinline__ int xcd_remap(int b, int nb) {
   const int per = nb >> 3, rem = nb & 7;
   const int xcd = b & 7, loc = b >> 3;
   return (xcd < rem) ? xcd * (per + 1) + loc : rem * (per + 1) + (xcd - rem) * per + loc;
+}
+
+// Workgroup -> logical item (tile + z chunk, tile fastest) for `order`:
+//   0: plain (the XCDs take tiles round-robin: with 8 tiles across, a tile column each);
+//   1: each XCD owns a contiguous run of the whole grid's items (its own z chunks);
+//   2: each XCD owns a contiguous run of the tiles of every z layer (`layer` items): a
+//      band of whole tile rows, while all XCDs stay on the same z chunk.
+// Every order is a bijection of [0, nb); nb is a multiple of layer.
+__device__ __forceinline__ int tile_order(int b, int nb, int layer, int order) {
+  if (order == 1) return xcd_remap(b, nb);
+  if (order == 2) {
+    const int z = b / layer;
+    return z * layer + xcd_remap(b - z * layer, layer);
+  }
+  return b;
 }
 
 template <int R, int TX, int TY>
@@ -253,7 +268,7 @@ void sweep_kernel(const SweepArgs<T, C> a) {
   using ST = SweepTile<T, C, KID, BY, RY, FAST, ROCS, RING, ND, BX, STRM, BUF, PF>;
   __shared__ typename ST::Lds lds;
   const int nb = a.tiles_x * a.tiles_y * a.nzc;
-  const int lb = a.xcd_order ? xcd_remap(blockIdx.x, nb) : (int)blockIdx.x;
+  const int lb = tile_order(blockIdx.x, nb, a.tiles_x * a.tiles_y, a.xcd_order);
   const int tile_x = lb % a.tiles_x;
   const int tile_y = (lb / a.tiles_x) % a.tiles_y;
   const int zc = lb / (a.tiles_x * a.tiles_y);

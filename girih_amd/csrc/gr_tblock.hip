@@ -65,7 +65,7 @@ __global__ __launch_bounds__(NT) void tb_kernel(const TbArgs<T, C> a) {
   __shared__ T lds[TL][PL];
 
   const int ntile = a.tiles_x * a.tiles_y;
-  const int lb = a.xcd_order ? xcd_remap(blockIdx.x, ntile * a.nzc) : (int)blockIdx.x;
+  const int lb = tile_order(blockIdx.x, ntile * a.nzc, ntile, a.xcd_order);
   const int tile = lb % ntile;
   const int tx0 = R + (tile % a.tiles_x) * TXO;  // tile origin (girih interior indices)
   const int ty0 = R + (tile / a.tiles_x) * TYO;

@@ -17,7 +17,7 @@ __global__ __launch_bounds__(64 * BX * BY) void wave2_kernel(const W2Args<T, C> 
   using WT = Wave2Tile<T, C, FAST, ROCS, STRM, BX, BY, RY2>;
   __shared__ typename WT::Lds lds;
   const int ntile = a.tiles_x * a.tiles_y;
-  const int lb = a.xcd_order ? xcd_remap(blockIdx.x, ntile * a.nzc) : (int)blockIdx.x;
+  const int lb = tile_order(blockIdx.x, ntile * a.nzc, ntile, a.xcd_order);
   WT::run(a, lb % ntile, lb / ntile, lds);
 }
 

@@ -79,7 +79,7 @@ typedef struct gr_stats {
   int32_t pitch_x, rows_y, planes_z; /* device layout of one field */
   int32_t x_offset;          /* element offset of logical x = 0 within a row */
   int32_t tile_variant;      /* gr_set_config tile variant in use */
-  int32_t xcd_order;         /* 1: workgroups renumbered so each XCD owns a contiguous tile range */
+  int32_t xcd_order;         /* workgroup order (gr_set_xcd_order): 0 plain, 1 or 2 XCD-contiguous */
 } gr_stats;
 
 /* ---- Level 1: time stepper (replaces TSList[ts].func, src/wrappers.h:28-36) ---- */
@@ -156,7 +156,10 @@ int gr_init_coef_uniform(gr_handle *h, uint64_t seed, double lo0, double hi0, do
  * otherwise the stepper uses single sweeps.  Values < 0 leave a setting unchanged.  Results
  * never depend on the configuration. */
 int gr_set_config(gr_handle *h, int tile_variant, int zchunk, int t_block);
-/* Workgroup order of the sweep kernels: 1 = XCD-contiguous tile ranges, 0 = plain. */
+/* Workgroup order of the sweep kernels and fused passes: 0 = plain (the XCDs take tiles
+ * round-robin); 1 = each XCD owns a contiguous range of all tiles and z chunks; 2 = each
+ * XCD owns a contiguous band of the tiles of every z chunk, while all XCDs work on the
+ * same chunk.  Results never depend on the order. */
 int gr_set_xcd_order(gr_handle *h, int xcd_order);
 /* Cache policy of the sweeps and fused passes (no counterpart in girih): mode -1
  * (default) uses non-temporal loads and stores for the data a step touches once when
@@ -170,9 +173,10 @@ int gr_set_streaming(gr_handle *h, int mode);
  * launch-bound grids).  Results are identical to eager launches. */
 int gr_set_graph(gr_handle *h, int on);
 /* Autotuner (replaces girih's CPU tile/thread-group search, diamond_utils.c:691-847):
- * times every tile variant x z-chunk x fused depth candidate for `steps` steps on
- * scratch copies of the fields, keeps the fastest, and reports it in *best (may be
- * NULL).  The problem's own fields and step parity are left untouched. */
+ * times every tile variant x workgroup order x z-chunk x fused depth candidate for
+ * `steps` steps on scratch copies of the fields, times the four fastest single-sweep
+ * candidates again over 4x as many steps, keeps the fastest, and reports it in *best
+ * (may be NULL).  The problem's own fields and step parity are left untouched. */
 int gr_autotune(gr_handle *h, int steps, gr_stats *best);
 
 /* Global reductions in FP64 over the interior cells: field fa of handle a against

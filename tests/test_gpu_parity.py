@@ -146,7 +146,7 @@ def test_every_tile_variant_bitexact(kernel, prec, strm):
         assert nvar == (17 if kernel == 0 else 16 if kernel in (1, 5) else 1)
         st.set_streaming(strm)
         for var in range(nvar):
-            for order, zc in ((0, 0), (1, 5)):
+            for order, zc in ((0, 0), (1, 5), (2, 3)):
                 print("variant", kernel, prec, strm, var, order, zc, flush=True)
                 st.upload(_pad(u), _pad(v), _pad(roc2) if roc2 is not None else None, _pad_coef(coef, kernel))
                 st.set_config(var, zc, 1)
@@ -193,7 +193,7 @@ def test_fast_every_variant_bitexact_vs_oracle(roc2_kind, strm):
     with gpu.Stepper(kernel, nx, ny, nz, dtype=gpu.GR_F32, flags=flags, roc2=0.13, coef=WAVE_COEF) as st:
         st.set_streaming(strm)
         for var in range(_num_variants(st)):
-            for order, zc in ((0, 0), (1, 5)):
+            for order, zc in ((0, 0), (1, 5), (2, 3)):
                 print("fast variant", roc2_kind, strm, var, order, zc, flush=True)
                 st.upload(_pad(u), _pad(v), _pad(roc2) if roc2 is not None else None)
                 st.set_source(*pt, src)

@@ -168,7 +168,7 @@ def test_fused_zchunks_bitexact_vs_oracle(kernel, depth, zchunk, prec):
     u, v, roc2, coef = _random_problem(kernel, nx, ny, nz, dt, 500 + kernel)
     outs = []
     with gpu.Stepper(kernel, nx, ny, nz, dtype=dtype, coef=_consts(coef, kernel)) as st:
-        for order in (0, 1):  # plain and XCD-contiguous workgroup order
+        for order in (0, 1, 2):  # plain, XCD-contiguous and per-layer XCD-band workgroup order
             st.upload(_pad(u), _pad(v), _pad(roc2) if roc2 is not None else None, _pad_coef(coef, kernel))
             st.set_config(zchunk=zchunk, t_block=depth)
             st.set_xcd_order(order)
