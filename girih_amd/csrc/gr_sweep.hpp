@@ -16,8 +16,9 @@
 //     arrays) is issued one plane ahead, so HBM latency overlaps the arithmetic;
 //   * x is the 64-lane wavefront axis: every load/store of a row is one
 //     coalesced 256 B (FP32) / 512 B (FP64) transaction;
-//   * workgroups are renumbered so that each XCD processes a contiguous run of
-//     tiles: tiles that share halos run on the same XCD and hit its L2.
+//   * workgroups may be renumbered (tile_order) so that each XCD processes a
+//     contiguous run of tiles: tiles that share halos run on the same XCD and hit
+//     its L2.
 //
 // Arithmetic modes:
 //   * reference order (default): the association and operand order of girih's
