@@ -294,6 +294,8 @@ class Stepper:
         check(lib().gr_set_config(self._h, int(tile_variant), int(zchunk), int(t_block)))
 
     def set_xcd_order(self, order):
+        """Workgroup order: 0 plain, 1 XCD-contiguous runs of the whole grid, 2 XCD bands
+        of tile rows within each z chunk (gr_set_xcd_order)."""
         check(lib().gr_set_xcd_order(self._h, int(order)))
 
     def set_streaming(self, mode):
