@@ -390,3 +390,15 @@ def test_set_config_rejects_unavailable_depth():
             with pytest.raises(gpu.GirihError) as ei:
                 st.set_config(t_block=depth)
             assert ei.value.code == gpu.GR_EUNSUPPORTED
+
+
+def test_set_xcd_order_accepts_three_orders():
+    with gpu.Stepper(0, 16, 16, 16) as st:
+        for order in (0, 1, 2):
+            st.set_xcd_order(order)
+            assert st.stats()["xcd_order"] == order
+        for bad in (-1, 3):
+            with pytest.raises(gpu.GirihError) as ei:
+                st.set_xcd_order(bad)
+            assert ei.value.code == gpu.GR_EINVAL
+        assert st.stats()["xcd_order"] == 2  # a rejected order leaves the setting unchanged
