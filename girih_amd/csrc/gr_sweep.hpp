@@ -433,7 +433,9 @@ __device__ __forceinline__ void SweepTile<T, C, KID, BY, RY, FAST, ROCS, RING, N
           uo[m] = uon[m];
           r2[m] = r2n[m];
         }
-        load_u0(min(k + 1, zlast), uon, r2n);
+        // plane k+1, kept inside this workgroup's chunk: the next chunk's first plane is
+        // being written by another workgroup (the value loaded past the end is unused)
+        load_u0(min(k + 1, z1 - 1), uon, r2n);
       } else {
         load_u0(k, uo, r2);
       }
