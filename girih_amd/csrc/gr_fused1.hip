@@ -177,6 +177,11 @@ __global__ __launch_bounds__(NT) void jacobi_tb_kernel(const J1Args<T> a) {
 
 template <typename T, int TL, int TX, int TY, int NT, bool STRM>
 static hipError_t launch_j1(J1Args<T> a, hipStream_t st) {
+  if (g_fused_tile_probe) {
+    g_fused_tile_probe[0] = TX;
+    g_fused_tile_probe[1] = TY;
+    return hipSuccess;
+  }
   a.tiles_x = (a.nx + TX - 1) / TX;
   a.tiles_y = (a.ny + TY - 1) / TY;
   a.zchunk = (a.zchunk <= 0 || a.zchunk > a.nz) ? a.nz : a.zchunk;

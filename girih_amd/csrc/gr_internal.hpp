@@ -41,6 +41,12 @@ int set_errorf(int code, const char *fmt, ...) __attribute__((format(printf, 2, 
 // One sweep u <- f(v); kernel 0 reads u^(n-1) from uold when given (else from u).
 hipError_t sweep_fields(const Problem &P, void *u, const void *v, const void *src, long long src_idx,
                         const LaunchCfg &lc, hipStream_t st, const void *uold = nullptr);
+// While set (only inside fused_tile), the fused-pass launchers store their output tile
+// (x, y cells) here and return without launching.
+extern thread_local int *g_fused_tile_probe;
+// Output tile of the fused pass of depth tl that tblock_pass would launch for P and lc;
+// false where there is no such pass.
+bool fused_tile(const Problem &P, int tl, const LaunchCfg &lc, int *tx, int *ty);
 hipError_t tblock_pass(const Problem &P, int tl, const void *X, const void *Y, void *ZO, void *ZN, const void *src,
                        long long src_idx, const LaunchCfg &lc, hipStream_t st);
 int tb_depths(int kid, int dtype, int *out, int max);

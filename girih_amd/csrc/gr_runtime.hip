@@ -669,7 +669,9 @@ extern "C" int gr_get_stats(gr_handle *h, gr_stats *out) {
   if (!h || !out) return fail(GR_EINVAL, "gr_get_stats: null argument");
   gr_stats s = h->stats;
   s.t_block = h->lc.t_block;
-  tile_of_variant(h->P.kernel, h->P.dtype, h->lc.variant, &s.tile_x, &s.tile_y);
+  // the fused pass's tile when the configuration fuses steps, else the sweep's
+  if (h->lc.t_block < 2 || !fused_tile(h->P, h->lc.t_block, h->lc, &s.tile_x, &s.tile_y))
+    tile_of_variant(h->P.kernel, h->P.dtype, h->lc.variant, &s.tile_x, &s.tile_y);
   s.zchunk = h->lc.zchunk;
   s.parity = h->parity;
   s.pitch_x = h->P.pitch;

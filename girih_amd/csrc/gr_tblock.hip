@@ -50,7 +50,8 @@ struct TbArgs {
 };
 
 // CP: level t+1 takes plane p's coefficients from level t one step earlier (same lane
-// and column) instead of re-reading them; costs TL x 7 registers per column.
+// and column) instead of re-reading them; costs TL x 7 registers per column, or TL x 4
+// with bf16 coefficients, which are handed on as stored, two to a register.
 template <typename T, typename C, int KID, int NT, int TXO, int TYO, int TL, bool CP>
 __global__ __launch_bounds__(NT) void tb_kernel(const TbArgs<T, C> a) {
   static_assert(KID == 5, "kernel 5 only");
@@ -156,8 +157,12 @@ __global__ __launch_bounds__(NT) void tb_kernel(const TbArgs<T, C> a) {
 
   // cs[t-1][j]: the coefficients level t uses in this step.  Level t+1 is live at
   // (step k0+1, column j) only where level t was live at (k0, j) on the same plane p =
-  // k0 - t: level regions shrink with t and level t+1 starts R planes later.
-  T cs[CP ? TL : 1][NCOL][7];
+  // k0 - t: level regions shrink with t and level t+1 starts R planes later.  bf16
+  // coefficients are handed on as stored, two to a register (PK).
+  constexpr bool PK = CP && std::is_same<C, __hip_bfloat16>::value;
+  constexpr int CW = PK ? 4 : 7;  // registers per column and level
+  using CS = typename std::conditional<PK, unsigned, T>::type;
+  CS cs[CP ? TL : 1][NCOL][CW];
 
   for (int kb = kfirst; kb < klast; kb += Q) {
 #pragma unroll
@@ -170,7 +175,7 @@ __global__ __launch_bounds__(NT) void tb_kernel(const TbArgs<T, C> a) {
 #pragma unroll
           for (int j = 0; j < NCOL; ++j)
 #pragma unroll
-            for (int m = 0; m < 7; ++m) cs[t][j][m] = cs[t - 1][j][m];
+            for (int m = 0; m < CW; ++m) cs[t][j][m] = cs[t - 1][j][m];
       }
       // slot of plane p: ((p mod Q) + Q) mod Q; k0 - kfirst = kb - kfirst + ph and
       // (kb - kfirst) is a multiple of Q, so plane k0 - R*t sits in slot
@@ -222,7 +227,24 @@ __global__ __launch_bounds__(NT) void tb_kernel(const TbArgs<T, C> a) {
               const T v0 = zq[R];
               {  // stencils.c:193-201; coefficients re-read by every level stay cached
                 const C *cp = a.coef + (long long)p * a.sz + off[j] / sizeof(T);
-                if constexpr (CP) {
+                if constexpr (PK) {  // the bits of a bf16 value are the top half of its FP32 value
+                  if (t == 1) {
+                    const unsigned short *cb = reinterpret_cast<const unsigned short *>(cp);
+                    unsigned short h[7];
+#pragma unroll
+                    for (int m = 0; m < 7; ++m) h[m] = cb[m * a.cstride];
+#pragma unroll
+                    for (int w = 0; w < 3; ++w) cs[0][j][w] = (unsigned)h[2 * w] | ((unsigned)h[2 * w + 1] << 16);
+                    cs[0][j][3] = h[6];
+                  }
+                  const unsigned *w = cs[t - 1][j];
+                  T c[7];
+#pragma unroll
+                  for (int m = 0; m < 7; ++m)
+                    c[m] = __uint_as_float((m & 1) ? (w[m >> 1] & 0xffff0000u) : (w[m >> 1] << 16));
+                  out = c[0] * v0 + c[1] * xn(-1) + c[2] * xn(1) + c[3] * yn(-1) + c[4] * yn(1) + c[5] * zq[0] +
+                        c[6] * zq[2];
+                } else if constexpr (CP) {
                   if (t == 1) {
 #pragma unroll
                     for (int m = 0; m < 7; ++m) cs[0][j][m] = ld_coef<T, C, false>(cp + m * a.cstride);
@@ -272,6 +294,11 @@ __global__ __launch_bounds__(NT) void tb_kernel(const TbArgs<T, C> a) {
 
 template <typename T, typename C, int KID, int NT, int TXO, int TYO, int TL, bool CP = false>
 static hipError_t launch_tb(const TbArgs<T, C> &a0, hipStream_t st) {
+  if (g_fused_tile_probe) {
+    g_fused_tile_probe[0] = TXO;
+    g_fused_tile_probe[1] = TYO;
+    return hipSuccess;
+  }
   TbArgs<T, C> a = a0;
   a.tiles_x = (a.nx + TXO - 1) / TXO;
   a.tiles_y = (a.ny + TYO - 1) / TYO;
@@ -301,31 +328,38 @@ int tb_depths(int kid, int dtype, int *out, int max) {
 }
 
 // Tiles per kernel and precision, sized so the per-lane queues (levels x columns
-// x (2R+1) words) stay in registers without spilling.
+// x (2R+1) words) and the coefficient hand-off stay in registers without spilling.
 template <typename T, typename C>
 static hipError_t tb_dispatch(int kid, int tl, const TbArgs<T, C> &a, hipStream_t st) {
   constexpr bool F32 = std::is_same<T, float>::value;
-  constexpr int TYS = F32 ? 32 : 16;  // 64 x TYS output tile
-  // depth 2 hands the coefficients from level 1 to level 2 (2 x 7 values per column
-  // held in registers, no spills); at depths 4 and 6 the hand-off would spill, so
-  // those levels re-read them
+  constexpr bool B16 = std::is_same<C, __hip_bfloat16>::value;
   if (kid == 5) {
-    // FP32: 896 lanes give 3 columns per lane (66 x 34 level-1 cells), 86 VGPRs and 5
-    // waves per SIMD, against 5 columns, 139 VGPRs and 3 waves with 512 lanes.
-    // FP64: 640 lanes give 2 columns per lane (66 x 18 level-1 cells), 112 VGPRs and 4
-    // waves per SIMD, against 3 columns, 156 VGPRs and 3 waves with 512 lanes
+    // depth 2, coefficient hand-off. FP32: 896 lanes give 3 columns per lane (66 x 34
+    // level-1 cells), 86 VGPRs and 5 waves per SIMD, against 5 columns, 139 VGPRs and
+    // 3 waves with 512 lanes.  FP64: 640 lanes give 2 columns per lane (66 x 18
+    // level-1 cells), 112 VGPRs and 4 waves per SIMD, against 3 columns, 156 VGPRs and
+    // 3 waves with 512 lanes
     if (tl == 2) {
       if constexpr (F32) return launch_tb<T, C, 5, 896, 64, 32, 2, true>(a, st);
       else return launch_tb<T, C, 5, 640, 64, 16, 2, true>(a, st);
     }
-    // FP32 coefficients at depth 4: 896 lanes, 3 columns per lane (70 x 38 level-1 cells),
-    // 70 VGPRs (512 lanes: 6 columns, 119 VGPRs); with bf16 coefficients 512 lanes measured
-    // faster, and at depth 6 1024 lanes were slower
+    // depth 4, coefficient hand-off on 1024 lanes (512^3, 256-plane chunks, one box):
+    //   bf16 coefficients, 64 x 32 tile (3 columns per lane, 117 VGPRs): 388 GLUP/s,
+    //   against 274 re-reading them on 512 lanes;
+    //   FP32 coefficients, 64 x 16 tile (2 columns, 110 VGPRs): 284 against 206 (896
+    //   lanes re-reading them) and 221 at depth 2; a 64 x 32 tile spills.
+    // FP64 re-reads them (seven FP64 values per column and level do not fit).
     if (tl == 4) {
-      if constexpr (std::is_same<C, float>::value) return launch_tb<T, C, 5, 896, 64, 32, 4>(a, st);
-      else return launch_tb<T, C, 5, 512, 64, TYS, 4>(a, st);
+      if constexpr (B16) return launch_tb<T, C, 5, 1024, 64, 32, 4, true>(a, st);
+      else if constexpr (F32) return launch_tb<T, C, 5, 1024, 64, 16, 4, true>(a, st);
+      else return launch_tb<T, C, 5, 512, 64, 16, 4>(a, st);
     }
-    if (tl == 6) return launch_tb<T, C, 5, 512, 64, 16, 6>(a, st);
+    // depth 6: bf16 with the hand-off, 1024 lanes, 64 x 16 (110 VGPRs): 322 against 227;
+    // the FP32 hand-off spills at this depth
+    if (tl == 6) {
+      if constexpr (B16) return launch_tb<T, C, 5, 1024, 64, 16, 6, true>(a, st);
+      else return launch_tb<T, C, 5, 512, 64, 16, 6>(a, st);
+    }
   }
   return hipErrorInvalidValue;
 }
@@ -350,6 +384,19 @@ static hipError_t tb_pass_t(const Problem &P, int tl, const void *Y, void *ZN, c
   a.src_k = src_idx < 0 ? -1 : (int)(src_idx / sz);
   a.src_off = src_idx < 0 ? 0 : (int)(src_idx % sz);
   return tb_dispatch<T, C>(P.kernel, tl, a, st);
+}
+
+thread_local int *g_fused_tile_probe = nullptr;
+
+bool fused_tile(const Problem &P, int tl, const LaunchCfg &lc, int *tx, int *ty) {
+  int t[2] = {0, 0};
+  g_fused_tile_probe = t;
+  const hipError_t e = tblock_pass(P, tl, nullptr, nullptr, nullptr, nullptr, nullptr, -1, lc, nullptr);
+  g_fused_tile_probe = nullptr;
+  if (e != hipSuccess || t[0] <= 0) return false;
+  *tx = t[0];
+  *ty = t[1];
+  return true;
 }
 
 // One fused pass of tl levels.  Kernels 1 and 5: Y -> ZN (X, ZO unused).

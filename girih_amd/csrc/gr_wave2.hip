@@ -24,6 +24,11 @@ __global__ __launch_bounds__(64 * BX * BY) void wave2_kernel(const W2Args<T, C> 
 template <typename T, typename C, bool FAST, bool ROCS, bool STRM, int BX, int BY, int RY2>
 static hipError_t launch_w2(W2Args<T, C> a, hipStream_t st) {
   constexpr int TX = 64 * BX, TY = BY * RY2;
+  if (g_fused_tile_probe) {
+    g_fused_tile_probe[0] = TX;
+    g_fused_tile_probe[1] = TY;
+    return hipSuccess;
+  }
   a.tiles_x = (a.nx + TX - 1) / TX;
   a.tiles_y = (a.ny + TY - 1) / TY;
   a.zchunk = (a.zchunk <= 0 || a.zchunk > a.nz) ? a.nz : a.zchunk;

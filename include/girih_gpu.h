@@ -72,7 +72,7 @@ typedef struct gr_stats {
   int64_t last_updates;      /* lattice-site updates of the last gr_step */
   int32_t last_launches;     /* kernel launches of the last gr_step */
   int32_t t_block;           /* fused time steps per pass in use */
-  int32_t tile_x, tile_y;    /* output tile of the sweep kernels */
+  int32_t tile_x, tile_y;    /* output tile of the sweep kernel, or of the fused pass when t_block > 1 */
   int32_t zchunk;            /* z planes streamed per workgroup */
   int32_t parity;            /* next application writes U1 (0) or U2 (1) */
   int64_t device_bytes;      /* HBM held by the handle */

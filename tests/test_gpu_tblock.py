@@ -402,3 +402,18 @@ def test_set_xcd_order_accepts_three_orders():
                 st.set_xcd_order(bad)
             assert ei.value.code == gpu.GR_EINVAL
         assert st.stats()["xcd_order"] == 2  # a rejected order leaves the setting unchanged
+
+
+@pytest.mark.parametrize("kernel,dtype,flags,depth,tile", [
+    (5, gpu.GR_F32, 0, 4, [64, 16]), (5, gpu.GR_F32, gpu.GR_COEF_BF16, 4, [64, 32]),
+    (5, gpu.GR_F32, gpu.GR_COEF_BF16, 6, [64, 16]), (5, gpu.GR_F32, 0, 2, [64, 32]),
+    (5, gpu.GR_F64, 0, 2, [64, 16]), (1, gpu.GR_F32, 0, 2, [128, 32]), (1, gpu.GR_F64, 0, 3, [64, 16]),
+    (0, gpu.GR_F32, gpu.GR_FAST | gpu.GR_ROC2_SCALAR, 2, [64, 16])])
+def test_stats_report_the_fused_pass_tile(kernel, dtype, flags, depth, tile):
+    with gpu.Stepper(kernel, 40, 40, 24, dtype=dtype, flags=flags) as st:
+        st.set_config(13, 0, 1)
+        s = st.stats()
+        assert [s["tile_x"], s["tile_y"]] == [128, 16] and s["t_block"] == 1
+        st.set_config(t_block=depth)
+        s = st.stats()
+        assert [s["tile_x"], s["tile_y"]] == tile and s["t_block"] == depth
