@@ -347,12 +347,13 @@ static hipError_t tb_dispatch(int kid, int tl, const TbArgs<T, C> &a, hipStream_
     //   bf16 coefficients, 64 x 32 tile (3 columns per lane, 117 VGPRs): 388 GLUP/s,
     //   against 274 re-reading them on 512 lanes;
     //   FP32 coefficients, 64 x 16 tile (2 columns, 110 VGPRs): 284 against 206 (896
-    //   lanes re-reading them) and 221 at depth 2; a 64 x 32 tile spills.
-    // FP64 re-reads them (seven FP64 values per column and level do not fit).
+    //   lanes re-reading them) and 221 at depth 2; a 64 x 32 tile spills;
+    //   FP64, 64 x 8 tile (1 column, 110 VGPRs): 128 against 79 re-reading them on 512
+    //   lanes and 104 at depth 2; 896 lanes (2 columns) spill.
     if (tl == 4) {
       if constexpr (B16) return launch_tb<T, C, 5, 1024, 64, 32, 4, true>(a, st);
       else if constexpr (F32) return launch_tb<T, C, 5, 1024, 64, 16, 4, true>(a, st);
-      else return launch_tb<T, C, 5, 512, 64, 16, 4>(a, st);
+      else return launch_tb<T, C, 5, 1024, 64, 8, 4, true>(a, st);
     }
     // depth 6: bf16 with the hand-off, 1024 lanes, 64 x 16 (110 VGPRs): 322 against 227;
     // the FP32 hand-off spills at this depth

@@ -100,10 +100,11 @@ def test_smallest_grids_bitexact(kernel):
 
 @pytest.mark.parametrize("kernel,n,prec,depth", [(1, 512, "sp", 4), (1, 512, "dp", 3), (0, 1024, "sp", 2),
                                                   (5, 512, "sp", 4), (5, 512, "dp", 2), (5, 512, "sp", 2),
-                                                  (5, 512, "bf16", 2), (5, 512, "bf16", 4), (5, 512, "bf16", 6)])
+                                                  (5, 512, "bf16", 2), (5, 512, "bf16", 4), (5, 512, "bf16", 6),
+                                                  (5, 512, "dp", 4)])
 def test_fused_equals_sweeps_at_benchmark_size(kernel, n, prec, depth):
     """Includes the passes the autotuner picks at these sizes: kernel 5 with the
-    coefficient hand-off at depth 4 (FP32 and bf16 coefficients, 1024 lanes) and at
+    coefficient hand-off at depth 4 (FP32, FP64 and bf16 coefficients, 1024 lanes) and at
     depth 2 (FP32 on 896 lanes, FP64 on 640 lanes)."""
     dtype = gpu.GR_F64 if prec == "dp" else gpu.GR_F32
     if kernel == 0:

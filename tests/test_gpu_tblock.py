@@ -407,7 +407,8 @@ def test_set_xcd_order_accepts_three_orders():
 @pytest.mark.parametrize("kernel,dtype,flags,depth,tile", [
     (5, gpu.GR_F32, 0, 4, [64, 16]), (5, gpu.GR_F32, gpu.GR_COEF_BF16, 4, [64, 32]),
     (5, gpu.GR_F32, gpu.GR_COEF_BF16, 6, [64, 16]), (5, gpu.GR_F32, 0, 2, [64, 32]),
-    (5, gpu.GR_F64, 0, 2, [64, 16]), (1, gpu.GR_F32, 0, 2, [128, 32]), (1, gpu.GR_F64, 0, 3, [64, 16]),
+    (5, gpu.GR_F64, 0, 2, [64, 16]), (5, gpu.GR_F64, 0, 4, [64, 8]), (1, gpu.GR_F32, 0, 2, [128, 32]),
+    (1, gpu.GR_F64, 0, 3, [64, 16]),
     (0, gpu.GR_F32, gpu.GR_FAST | gpu.GR_ROC2_SCALAR, 2, [64, 16])])
 def test_stats_report_the_fused_pass_tile(kernel, dtype, flags, depth, tile):
     with gpu.Stepper(kernel, 40, 40, 24, dtype=dtype, flags=flags) as st:
