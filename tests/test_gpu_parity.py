@@ -158,6 +158,30 @@ def test_every_tile_variant_bitexact(kernel, prec, strm):
                 np.testing.assert_array_equal(U2[:, :, :nnx], vo, err_msg="variant %d" % var)
 
 
+@pytest.mark.parametrize("kernel", [0, 1, 5])
+def test_tiny_grids_every_variant_bitexact(kernel):
+    """Grids smaller than every tile, with single-cell dimensions and one-plane z chunks,
+    in every tile variant and workgroup order: the clamped loads, the operand prefetch and
+    the workgroup maps must still give the oracle's result bit for bit."""
+    for (nx, ny, nz) in ((1, 1, 1), (2, 3, 1), (3, 1, 2), (1, 5, 3)):
+        u, v, roc2, coef = _random_problem(kernel, nx, ny, nz, np.float64, 300 + nx + 7 * ny + kernel)
+        uo, vo = u.copy(), v.copy()
+        orc.run(kernel, coef, uo, vo, roc2, nt=2)
+        orc.sweep(kernel, coef, uo, vo, roc2)
+        with gpu.Stepper(kernel, nx, ny, nz, dtype=gpu.GR_F64, coef=_consts(coef, kernel)) as st:
+            for var in range(_num_variants(st)):
+                for order, zc in ((0, 0), (2, 1)):
+                    print("tiny", kernel, (nx, ny, nz), var, order, zc, flush=True)
+                    st.upload(_pad(u), _pad(v), _pad(roc2) if roc2 is not None else None, _pad_coef(coef, kernel))
+                    st.set_config(var, zc, 1)
+                    st.set_xcd_order(order)
+                    st.step(3)
+                    U1, U2 = st.download(_pad(u).shape)
+                    nnx = u.shape[2]
+                    np.testing.assert_array_equal(U1[:, :, :nnx], uo, err_msg="variant %d" % var)
+                    np.testing.assert_array_equal(U2[:, :, :nnx], vo, err_msg="variant %d" % var)
+
+
 WAVE_W = [-205.0 / 72, 8.0 / 5, -1.0 / 5, 8.0 / 315, -1.0 / 560]
 WAVE_COEF = [3 * WAVE_W[0]] + WAVE_W[1:5]
 
