@@ -318,6 +318,32 @@ def test_fused_tiny_grids_bitexact(kernel, depth):
             np.testing.assert_array_equal(U2[:, :, :u.shape[2]], v, err_msg="%s %s" % ((nx, ny, nz), prec))
 
 
+@pytest.mark.parametrize("depth", [2, 4, 6])
+def test_fused_bf16_coefficients_tiny_grids_bitexact(depth):
+    """The bf16 coefficient hand-off (two values to a register) on grids smaller than one
+    tile and than the pipeline depth: bit-identical to the FP32 oracle on the rounded
+    coefficients."""
+    kernel = 5
+    for (nx, ny, nz) in ((1, 1, 1), (2, 3, 1), (3, 1, 2), (5, 4, 3)):
+        print("fused bf16 tiny", depth, (nx, ny, nz), flush=True)
+        u, v, roc2, coef = _random_problem(kernel, nx, ny, nz, np.float32, 90 + nx + 3 * ny)
+        coef = _bf16_round(coef)
+        napply = 2 * depth + 1
+        with gpu.Stepper(kernel, nx, ny, nz, dtype=gpu.GR_F32, flags=gpu.GR_COEF_BF16) as st:
+            st.upload(_pad(u), _pad(v), None, _pad_coef(coef, kernel))
+            st.set_config(zchunk=1 if nz > 1 else 0, t_block=depth)
+            st.step(napply)
+            assert st.stats()["last_launches"] < napply
+            U1, U2 = st.download(_pad(u).shape)
+        for a in range(napply):
+            if a % 2 == 0:
+                orc.sweep(kernel, coef, u, v, roc2)
+            else:
+                orc.sweep(kernel, coef, v, u, roc2)
+        np.testing.assert_array_equal(U1[:, :, :u.shape[2]], u, err_msg=str((nx, ny, nz)))
+        np.testing.assert_array_equal(U2[:, :, :u.shape[2]], v, err_msg=str((nx, ny, nz)))
+
+
 def test_fused_fast_fp32_equals_single_sweep_fast():
     """GR_FAST fused passes use the same arithmetic as GR_FAST single sweeps."""
     nx, ny, nz = 80, 36, 24
