@@ -105,6 +105,21 @@ def test_profile_summary_numbers(tmp_path):
     assert "6.00 GB/s" not in out.read_text() and "12.00 B/LUP" in out.read_text()
 
 
+def test_committed_traffic_index_matches_its_summaries():
+    """profiles/traffic.json (what bench.py reads for roofline.traffic) points at committed
+    summaries: each entry's key ends in its launch configuration, its summary exists, names
+    that configuration in its command, and reports the same average launch time."""
+    prof = os.path.join(REPO, "profiles")
+    idx = json.load(open(os.path.join(prof, "traffic.json")))
+    assert idx
+    for key, e in idx.items():
+        assert key.endswith("@" + e["launch_config"]), key
+        text = open(os.path.join(prof, e["summary"])).read()
+        assert e["launch_config"] in text, key
+        assert "average duration: %.2f us" % e["avg_launch_us"] in text, key
+        assert e["bytes_per_launch"] == pytest.approx(e["fetch_bytes"] + e["write_bytes"]), key
+
+
 def test_physics_helpers():
     c = physics.fd8_coef()
     assert abs(c[0] + 6 * sum(c[1:])) < 1e-12  # a constant field has zero Laplacian
